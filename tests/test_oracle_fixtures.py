@@ -1,0 +1,118 @@
+"""Pins the CPU oracle against the reference's own fixtures (test/data/*.dat, data/*.dat).
+
+The reference's C tests only print an error norm (no assertion); here each one
+becomes a thresholded test.  Residuals measured for this oracle are ~1e-15
+(L1-relative) on every fixture; the thresholds leave two orders of margin.
+Parameters are those of the reference test programs cited per test.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import l1rel, read_dat
+
+N, L, R = 32, 12.0, 7.5
+TOL = 1e-13
+
+
+@pytest.fixture(scope="module")
+def fgh():
+    c = np.complex128
+    return tuple(read_dat(n, c, (N, N)) for n in ("f.dat", "g.dat", "h.dat"))
+
+
+@pytest.fixture(scope="module")
+def WF():
+    W = np.stack([read_dat(f"W{i}.dat", shape=(N, N)) for i in range(4)])
+    return np.stack([O.r2c_fill(W[i]) for i in range(4)])
+
+
+def test_lgwt():  # test/lgwt_test.c:49-72
+    x, w = O.legendre_gauss(15, 0.5, 2.0)
+    assert np.abs(x - read_dat("x_ref.dat")).sum() + np.abs(w - read_dat("w_ref.dat")).sum() < 1e-14
+    # nodes are numpy's Gauss-Legendre nodes in descending order
+    xn, wn = np.polynomial.legendre.leggauss(15)
+    assert np.abs(x - ((0.5 * (1 - xn) + 2.0 * (1 + xn)) / 2)[::-1]).max() < 1e-14
+    assert np.abs(w - wn[::-1] * 0.75).max() < 1e-14
+
+
+def test_I1(fgh):  # test/I1_test.c:45-76, J=32
+    p1, p2 = O.fourier_I1(N, 32, L, R)
+    assert l1rel(O.I1integral(*fgh, p1, p2), read_dat("I1_ref.dat", np.complex128, (N, N))) < TOL
+
+
+def test_I2_custom_weights(fgh):  # test/I2_test.c:47-98, J=7, hand-supplied psi grids
+    wt = read_dat("weight.dat")
+    p1 = np.stack([read_dat(f"psiR1_{j + 1}.dat", shape=(N, N)) for j in range(7)])
+    p2 = np.stack([read_dat(f"psiR2_{j + 1}.dat", shape=(N, N)) * wt[j] for j in range(7)])
+    assert l1rel(O.I2integral(*fgh, p1, p2), read_dat("I2_ref.dat", np.complex128, (N, N))) < TOL
+
+
+def test_I2_fourier(fgh):  # test/I2_test2.c, J=32
+    p1, p2 = O.fourier_I2(N, 32, L, R)
+    assert l1rel(O.I2integral(*fgh, p1, p2), read_dat("I2_ref2.dat", np.complex128, (N, N))) < TOL
+
+
+def test_I3(fgh):  # test/I3_test.c, J=32, M=7
+    p1, p2 = O.fourier_I3(N, 32, L, R, 7)
+    assert l1rel(O.I3integral(*fgh, p1, p2), read_dat("I3_ref.dat", np.complex128, (N, N))) < TOL
+
+
+def test_I4(fgh):  # test/I4_test.c, J=32, I4 = I1integral with FourierI4 tables
+    p1, p2 = O.fourier_I4(N, 32, L, R)
+    assert l1rel(O.I1integral(*fgh, p1, p2), read_dat("I4_ref.dat", np.complex128, (N, N))) < TOL
+
+
+def test_Cc_Cd(WF):  # test/Cc_test.c (J=48), test/Cd_test.c (J=17, M=15)
+    q = O.Quad(N, 17, L, R, 15, J1=48)
+    Cc = O.CcInt(q, WF)
+    ref = np.stack([read_dat(f"Cc{i}_ref.dat", np.complex128, (N, N)) for i in range(4)])
+    assert l1rel(Cc, ref) < TOL
+    assert np.all(Cc[0] == 0)  # trace part of the commutator vanishes identically
+    Cd = O.CdInt(q, WF)
+    ref = np.stack([read_dat(f"Cd{i}_ref.dat", np.complex128, (N, N)) for i in range(4)])
+    assert l1rel(Cd, ref) < TOL
+
+
+@pytest.mark.slow
+def test_collision_step(WF):  # test/coll_step_test.c, J=M=32, dt=1e-3, Bext=0
+    q = O.Quad(N, 32, L, R, 32)
+    out = O.collision_step(q, WF, 1e-3)
+    ref = np.stack([read_dat(f"WFnext{i}_ref.dat", np.complex128, (N, N)) for i in range(4)])
+    assert l1rel(out, ref) < TOL
+
+
+@pytest.mark.slow
+def test_inhom_step_periodic():  # test/sim_step_test.c: 8 cells, J=16, M=32, h=0.08, dt=0.002
+    q = O.Quad(N, 16, L, R, 32)
+    W = read_dat("Wx.dat", shape=(8, 4, N, N))
+    out = O.inhom_step_periodic(q, W, 0.08, 0.002)
+    assert l1rel(out, read_dat("Wx_next_ref.dat", shape=(8, 4, N, N))) < TOL
+
+
+@pytest.mark.slow
+def test_trajectory_homogeneous_frame1():
+    """data/Wevolv_hom.dat was written by the reference C code itself (src/main_hom.c:40-50)."""
+    q = O.Quad(N, 32, L, R, 32)
+    ev = read_dat("Wevolv_hom.dat", shape=(101, 4, N, N))
+    out = O.simulation_homogeneous(q, ev[0], 1e-3, 2)
+    assert np.array_equal(out[0], ev[0])
+    assert l1rel(out[1], ev[1]) < TOL
+
+
+@pytest.mark.slow
+def test_trajectory_periodic_frame1():
+    """data/Wevolv_inhom_periodic.dat (bin/params_periodic_example.txt: J=M=32, h=0.1, dt=0.008)."""
+    q = O.Quad(N, 32, L, R, 32)
+    ev = read_dat("Wevolv_inhom_periodic.dat", shape=(5, 10, 4, N, N))
+    out = O.inhom_step_periodic(q, ev[0], 0.1, 0.008)
+    assert l1rel(out, ev[1]) < TOL
+
+
+def test_slope_limiters():  # test/slope_lim_test.c and slope_lim_test2.c
+    Un = read_dat("Un.dat")
+    assert np.array_equal(O.slope_limiter_periodic(0.08, 0.02, -1.2, Un), read_dat("Un1_periodic_ref.dat"))
+    assert np.array_equal(O.slope_limiter_dirichlet(0.08, 0.02, -1.2, Un), read_dat("Un1_Dirichlet_ref.dat"))
+    out = O.slope_limiter_maxwell(0.01, 0.005, [1.0, 0.5, -0.5, -1.0], 0.4, 0.7, 0.45,
+                                  [0.2, -np.pi / 4, 0, 0], [0, 0, 1 / np.e, -0.8], read_dat("UnM.dat"))
+    assert np.array_equal(out, read_dat("Un1_Maxwell_ref.dat"))
