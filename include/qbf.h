@@ -1,0 +1,95 @@
+/*
+ * qbf.h -- device-resident C API of the MI355X collision-plus-transport engine.
+ *
+ * This is the native layer under the drop-in reference headers (collision.h,
+ * integrals.h, simulation.h, finite_volume.h of cmendl/inhom-quantum-boltzmann-fourier-c,
+ * re-declared in this directory).  It has no counterpart in the reference: it
+ * exposes batched entry points over device memory so that a time loop keeps
+ * the state in HBM (the reference's entry points take host memory and one cell
+ * at a time).
+ *
+ * Layouts (all device pointers, double precision, complex = interleaved pair):
+ *   Fourier state  [ncell][4][N][N] complex   (wignerF_t per cell, include/wigner.h:47-50)
+ *   physical state [ncell][4][N][N] real      (wignerV_t per cell, include/wigner.h:35-41)
+ *   index within a grid: k + N*l, l = slow index (v_x), k = fast index (v_y).
+ * Every function returns 0 on success and a negative code on error; the message
+ * is available from qbf_last_error().  Calls are asynchronous on the engine's
+ * stream; qbf_sync() waits for completion.
+ */
+#ifndef QBF_H
+#define QBF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qbf_engine qbf_engine;
+
+enum { QBF_FP64 = 0, QBF_FP32 = 1 };              /* arithmetic of the convolution kernels */
+enum { QBF_CC = 1, QBF_CD = 2, QBF_C = 3 };        /* which collision operator */
+enum { QBF_TAB_I1 = 0, QBF_TAB_I4 = 1 };           /* I1-type table slots */
+
+/* Quadrature tables in the reference's host layout (include/quadrature.h:35-77), P = 2N:
+ *   I1/I4: psiR1[num][P*P] real, psiR2[num][N*N] real
+ *   I2:    psiR1[num][N*N] real, psiR2[num][N*N] real
+ *   I3:    psiR1[num][N*N] complex, psiR2[num][P*P] real */
+typedef struct {
+	unsigned numI1; const double *I1_psiR1, *I1_psiR2;
+	unsigned numI2; const double *I2_psiR1, *I2_psiR2;
+	unsigned numI3; const double *I3_psiR1, *I3_psiR2;
+	unsigned numI4; const double *I4_psiR1, *I4_psiR2;
+} qbf_quad_tables;
+
+const char *qbf_last_error(void);
+
+int  qbf_engine_create(int N, int precision, int maxCells, qbf_engine **out);
+void qbf_engine_destroy(qbf_engine *e);
+int  qbf_engine_N(const qbf_engine *e);
+int  qbf_engine_precision(const qbf_engine *e);
+/* run on an existing stream (e.g. torch.cuda.current_stream().cuda_stream); NULL = own stream */
+int  qbf_set_stream(qbf_engine *e, void *hip_stream);
+int  qbf_sync(qbf_engine *e);
+
+/* upload all four tables (host memory) */
+int qbf_set_quadrature(qbf_engine *e, const qbf_quad_tables *q);
+/* upload a single table set (host memory) */
+int qbf_set_i1_table(qbf_engine *e, int slot, unsigned num, const double *psiR1, const double *psiR2);
+int qbf_set_i2_table(qbf_engine *e, unsigned num, const double *psiR1, const double *psiR2);
+int qbf_set_i3_table(qbf_engine *e, unsigned num, const double *psiR1, const double *psiR2);
+
+/* out = base + cf * ( C_mode(W) + 2 (Bext x w)(Wcb) )   for ncell Fourier states.
+ * base, Wcb may be NULL (then 0 / no magnetic term); Bext is [ncell][3] (device). */
+int qbf_collision(qbf_engine *e, const double *dW, int ncell, int mode, double *dOut,
+	const double *dBase, const double *dWcb, const double *dBext, double cf);
+
+/* explicit midpoint collision step on ncell Fourier states (src/simulation.c:105-136) */
+int qbf_collision_step(qbf_engine *e, const double *dW, int ncell, double dt, const double *dBext, double *dWnext);
+
+/* single integrals on one N x N complex grid each (src/integrals.c) */
+int qbf_I1(qbf_engine *e, int slot, const double *df, const double *dg, const double *dh, double *dOut);
+int qbf_I2(qbf_engine *e, const double *df, const double *dg, const double *dh, double *dOut);
+int qbf_I3(qbf_engine *e, const double *df, const double *dg, const double *dh, double *dOut);
+
+/* physical real N x N grids <-> Fourier (scaled by 1/N^2; inverse: unnormalised, real part) */
+int qbf_phys_to_fourier(qbf_engine *e, const double *dPhys, int ngrid, double *dFourier);
+int qbf_fourier_to_phys(qbf_engine *e, const double *dFourier, int ngrid, double *dPhys);
+
+/* slope-limited transport, periodic in x, over ncell physical states (src/finite_volume.c:52-104) */
+int qbf_transport_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt, double *dWout);
+
+/* one Strang-split periodic step: half transport, collision, half transport (src/simulation.c:307-555) */
+int qbf_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
+	const double *dBext, double *dWnext);
+
+/* device memory helpers for host code that does not include HIP headers */
+void *qbf_malloc(size_t bytes);
+void  qbf_free(void *p);
+int   qbf_h2d(qbf_engine *e, void *dst, const void *src, size_t bytes);
+int   qbf_d2h(qbf_engine *e, void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

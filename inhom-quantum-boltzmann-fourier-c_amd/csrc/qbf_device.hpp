@@ -1,0 +1,275 @@
+// qbf_device.hpp -- device building blocks for batched 2-D FFT pipelines on gfx950.
+//
+// A P x P complex grid (row index l, column index k, element (l, k)) is owned by
+// one workgroup of T threads.  G = T / P threads share one "line" (a row or a
+// column); each thread holds E = P / G elements of its line in registers.
+// Four register layouts are used:
+//   R_i  line = row l,    thread p holds k = p + G*s          (s < E)  interleaved
+//   R_b  line = row l,    thread p holds k = E*brev_G(p) + u  (u < E)  blocked
+//   C_i  line = column k, thread p holds l = p + G*s
+//   C_b  line = column k, thread p holds l = E*brev_G(p) + u
+// A forward line FFT maps an interleaved layout to the blocked one (in-register
+// DFT of length E, twiddle, radix-2 butterflies across the G lanes of the line);
+// the backward line FFT maps blocked to interleaved.  Rows and columns are
+// exchanged through LDS (R_i -> C_b and C_b -> R_i only).  Grids that live in
+// global memory between kernels are stored in "native" order: element u of
+// thread t at offset u*T + t, so every load and store of a wave is coalesced.
+//
+// FFT sign convention: forward = exp(-2 pi i jk/n), backward = exp(+2 pi i jk/n),
+// both unnormalised (FFTW_FORWARD / FFTW_BACKWARD, as used by the reference
+// src/integrals.c:48-51).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qbf {
+
+template <typename R> struct cplx_sel;
+template <> struct cplx_sel<float> { using type = float2; };
+template <> struct cplx_sel<double> { using type = double2; };
+template <typename R> using cx = typename cplx_sel<R>::type;
+
+template <typename R>
+__device__ __forceinline__ cx<R> mk(R a, R b)
+{
+	cx<R> r;
+	r.x = a;
+	r.y = b;
+	return r;
+}
+template <typename R> __device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
+template <typename R> __device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
+template <typename R> __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b)
+{
+	return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename R> __device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b)
+{
+	return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename R> __device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
+template <typename R> __device__ __forceinline__ cx<R> czero() { return mk<R>(R(0), R(0)); }
+
+// cos / sin (2 pi j / 64), j < 64, correctly rounded double literals.
+struct Tw64 {
+	static constexpr double c[64] = {
+		1.0, 0.9951847266721969, 0.9807852804032304, 0.9569403357322088,
+		0.9238795325112867, 0.881921264348355, 0.8314696123025452, 0.773010453362737,
+		0.7071067811865476, 0.6343932841636455, 0.5555702330196023, 0.4713967368259978,
+		0.38268343236508984, 0.29028467725446233, 0.19509032201612833, 0.09801714032956077,
+		6.123233995736766e-17, -0.09801714032956065, -0.1950903220161282, -0.29028467725446216,
+		-0.3826834323650897, -0.4713967368259977, -0.555570233019602, -0.6343932841636454,
+		-0.7071067811865475, -0.773010453362737, -0.8314696123025453, -0.8819212643483549,
+		-0.9238795325112867, -0.9569403357322088, -0.9807852804032304, -0.9951847266721968,
+		-1.0, -0.9951847266721969, -0.9807852804032304, -0.9569403357322089,
+		-0.9238795325112868, -0.881921264348355, -0.8314696123025455, -0.7730104533627371,
+		-0.7071067811865477, -0.6343932841636459, -0.5555702330196022, -0.47139673682599786,
+		-0.38268343236509034, -0.29028467725446244, -0.19509032201612866, -0.09801714032956045,
+		-1.8369701987210297e-16, 0.09801714032956009, 0.1950903220161283, 0.29028467725446205,
+		0.38268343236509, 0.4713967368259976, 0.5555702330196018, 0.6343932841636456,
+		0.7071067811865474, 0.7730104533627367, 0.8314696123025452, 0.8819212643483548,
+		0.9238795325112865, 0.9569403357322088, 0.9807852804032303, 0.9951847266721969};
+	static constexpr double sn[64] = {
+		0.0, 0.0980171403295606, 0.19509032201612825, 0.29028467725446233,
+		0.3826834323650898, 0.47139673682599764, 0.5555702330196022, 0.6343932841636455,
+		0.7071067811865475, 0.773010453362737, 0.8314696123025452, 0.8819212643483549,
+		0.9238795325112867, 0.9569403357322089, 0.9807852804032304, 0.9951847266721968,
+		1.0, 0.9951847266721969, 0.9807852804032304, 0.9569403357322089,
+		0.9238795325112867, 0.881921264348355, 0.8314696123025455, 0.7730104533627371,
+		0.7071067811865476, 0.6343932841636455, 0.5555702330196022, 0.47139673682599786,
+		0.3826834323650899, 0.2902846772544624, 0.1950903220161286, 0.09801714032956083,
+		1.2246467991473532e-16, -0.09801714032956059, -0.19509032201612836, -0.2902846772544621,
+		-0.38268343236508967, -0.47139673682599764, -0.555570233019602, -0.6343932841636453,
+		-0.7071067811865475, -0.7730104533627367, -0.8314696123025452, -0.8819212643483549,
+		-0.9238795325112865, -0.9569403357322088, -0.9807852804032303, -0.9951847266721969,
+		-1.0, -0.9951847266721969, -0.9807852804032304, -0.9569403357322089,
+		-0.9238795325112866, -0.881921264348355, -0.8314696123025455, -0.7730104533627369,
+		-0.7071067811865477, -0.6343932841636459, -0.5555702330196022, -0.4713967368259979,
+		-0.3826834323650904, -0.2902846772544625, -0.19509032201612872, -0.0980171403295605};
+	static constexpr double s(int j) { return sn[j]; }
+};
+
+template <int E>
+__host__ __device__ constexpr int brev(int i)
+{
+	int r = 0;
+	for (int b = 1, rb = E >> 1; b < E; b <<= 1, rb >>= 1)
+		if (i & b) r |= rb;
+	return r;
+}
+
+template <int X>
+__host__ __device__ constexpr int ilog2c()
+{
+	int l = 0;
+	while ((1 << l) < X) l++;
+	return l;
+}
+
+// c * exp(SIGN * 2 pi i j / m) for compile-time (after unrolling) j, m <= 64.
+template <typename R, int SIGN>
+__device__ __forceinline__ cx<R> twmul(cx<R> c, int j, int m)
+{
+	if (j == 0) return c;
+	if (4 * j == m) return SIGN < 0 ? mk<R>(c.y, -c.x) : mk<R>(-c.y, c.x);
+	if (4 * j == 3 * m) return SIGN < 0 ? mk<R>(-c.y, c.x) : mk<R>(c.y, -c.x);
+	if (2 * j == m) return mk<R>(-c.x, -c.y);
+	const R cr = (R)Tw64::c[j * (64 / m)];
+	const R si = (R)(SIGN * Tw64::s(j * (64 / m)));
+	return mk<R>(c.x * cr - c.y * si, c.x * si + c.y * cr);
+}
+
+// In-register DFT of length E (power of two <= 64), natural order in and out.
+template <typename R, int E, int SIGN>
+__device__ __forceinline__ void dft(cx<R>* v)
+{
+	if constexpr (E > 1) {
+		cx<R> w[E];
+#pragma unroll
+		for (int i = 0; i < E; i++) w[i] = v[brev<E>(i)];
+#pragma unroll
+		for (int m = 2; m <= E; m <<= 1) {
+#pragma unroll
+			for (int b = 0; b < E; b += m) {
+#pragma unroll
+				for (int j = 0; j < m / 2; j++) {
+					const cx<R> t = twmul<R, SIGN>(w[b + j + m / 2], j, m);
+					const cx<R> a = w[b + j];
+					w[b + j] = cadd<R>(a, t);
+					w[b + j + m / 2] = csub<R>(a, t);
+				}
+			}
+		}
+#pragma unroll
+		for (int i = 0; i < E; i++) v[i] = w[i];
+	}
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m)
+{
+	return mk<R>(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
+}
+
+// Geometry of one P x P grid per workgroup of T threads.
+template <int P_, int T_>
+struct Geo {
+	static constexpr int P = P_;
+	static constexpr int T = T_;
+	static constexpr int G = T_ / P_;       // threads per line
+	static constexpr int E = P_ / G;        // elements per thread
+	static constexpr int PAD = 1;           // LDS row padding (elements)
+	static constexpr int PITCH = P_ + PAD;
+	static_assert(G * P_ == T_, "T must be a multiple of P");
+	static_assert(G * E == P_, "G must divide P");
+	static_assert(G <= 64 && (64 % G) == 0, "a line must sit inside one wavefront");
+};
+
+// Forward FFT of one line: interleaved layout in, blocked layout out.
+// twP[j] = (cos, sin)(2 pi j / P), j < P (in LDS).
+template <typename R, int P, int G, int E>
+__device__ __forceinline__ void line_fwd(cx<R>* v, int p, const cx<R>* twP)
+{
+	dft<R, E, -1>(v);
+	if constexpr (G > 1) {
+#pragma unroll
+		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], twP[p * u]);
+#pragma unroll
+		for (int h = G / 2; h >= 1; h >>= 1) {
+			const bool hi = (p & h) != 0;
+			const cx<R> w = twP[(p & (h - 1)) * (P / (2 * h))];
+#pragma unroll
+			for (int u = 0; u < E; u++) {
+				const cx<R> pv = shfl_xor_c<R>(v[u], h);
+				const cx<R> d = cmulc<R>(csub<R>(pv, v[u]), w);
+				const cx<R> s = cadd<R>(v[u], pv);
+				v[u] = hi ? d : s;
+			}
+		}
+	}
+}
+
+// Backward FFT of one line: blocked layout in, interleaved layout out.
+template <typename R, int P, int G, int E>
+__device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
+{
+	if constexpr (G > 1) {
+#pragma unroll
+		for (int h = 1; h < G; h <<= 1) {
+			const bool hi = (p & h) != 0;
+			const cx<R> w = twP[(p & (h - 1)) * (P / (2 * h))];
+#pragma unroll
+			for (int u = 0; u < E; u++) {
+				const cx<R> pv = shfl_xor_c<R>(v[u], h);
+				const cx<R> a = hi ? pv : v[u];
+				const cx<R> b = hi ? v[u] : pv;
+				const cx<R> wb = cmul<R>(b, w);
+				v[u] = hi ? csub<R>(a, wb) : cadd<R>(a, wb);
+			}
+		}
+#pragma unroll
+		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], twP[p * u]);
+	}
+	dft<R, E, +1>(v);
+}
+
+// LDS exchanges.  The grid is stored row-major with a padded pitch.
+template <typename R, class GE>
+struct Xpose {
+	cx<R>* g;
+	__device__ __forceinline__ int a(int row, int col) const { return row * GE::PITCH + col; }
+	// R_i (line = row) -> LDS
+	__device__ __forceinline__ void put_Ri(const cx<R>* v, int line, int p) const
+	{
+#pragma unroll
+		for (int s = 0; s < GE::E; s++) g[a(line, p + GE::G * s)] = v[s];
+	}
+	// LDS -> C_b (line = column)
+	__device__ __forceinline__ void get_Cb(cx<R>* v, int line, int p) const
+	{
+		const int r0 = GE::E * brev<GE::G>(p);
+#pragma unroll
+		for (int u = 0; u < GE::E; u++) v[u] = g[a(r0 + u, line)];
+	}
+	// C_b (line = column) -> LDS
+	__device__ __forceinline__ void put_Cb(const cx<R>* v, int line, int p) const
+	{
+		const int r0 = GE::E * brev<GE::G>(p);
+#pragma unroll
+		for (int u = 0; u < GE::E; u++) g[a(r0 + u, line)] = v[u];
+	}
+	// LDS -> R_i (line = row)
+	__device__ __forceinline__ void get_Ri(cx<R>* v, int line, int p) const
+	{
+#pragma unroll
+		for (int s = 0; s < GE::E; s++) v[s] = g[a(line, p + GE::G * s)];
+	}
+};
+
+// Fill the LDS twiddle table twP[j] = (cos, sin)(2 pi j / P) from a global table
+// of (cos, sin)(2 pi j / 256) in double.
+template <typename R, int P, int T>
+__device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
+{
+	for (int j = threadIdx.x; j < P; j += T) {
+		const double2 w = tw256[j * (256 / P)];
+		twP[j] = mk<R>((R)w.x, (R)w.y);
+	}
+}
+
+// Native (coalesced) storage of a register-resident grid.
+template <typename R, class GE>
+__device__ __forceinline__ void store_native(cx<R>* dst, const cx<R>* v)
+{
+#pragma unroll
+	for (int u = 0; u < GE::E; u++) dst[u * GE::T + threadIdx.x] = v[u];
+}
+template <typename R, class GE>
+__device__ __forceinline__ void load_native(cx<R>* v, const cx<R>* src)
+{
+#pragma unroll
+	for (int u = 0; u < GE::E; u++) v[u] = src[u * GE::T + threadIdx.x];
+}
+
+} // namespace qbf

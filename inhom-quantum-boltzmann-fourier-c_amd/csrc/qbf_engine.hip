@@ -1,0 +1,730 @@
+// qbf_engine.hip -- host side of the MI355X collision-plus-transport engine (C API in include/qbf.h).
+//
+// The engine owns the device copies of the quadrature tables (reordered into the
+// kernels' native layouts), the intermediate padded grids of one collision
+// evaluation, and launches the kernel sequence:
+//   What_c = FFT2(pad W_c)                                   k_padfft
+//   Ahat_g = FFT2(sum_i psi1_i .* IFFT2(Fhat .* FFT2(pad(Phi_i .* W_g))))   k_i1acc  (Cc, I4)
+//   S_i    = sum_q FFT2(pad(psi1_q W_i)) .* FFT2(pad(psi2_q W_i))           k_i2acc  (I2)
+//   acc_j  = sum_q sum_i eta_i F_iq .* K_q[What_i .* F_jq]                   k_padfft + k_i3acc (I3)
+//   C_j    = Extract(IFFT2( linear combination ))                           k_combine
+// The reassociation is exact: the I1 weights of the reference are duplicated over
+// one angle (src/quadrature.c:192-195, 213-234) so the J^2 terms are summed into J
+// groups with Phi_i = sum_j psiR2_ij; the final convolution and Extract of every
+// integral are linear and are hoisted out of the term sums.
+#include "qbf_kernels.hip"
+#include "../../include/qbf.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdarg.h>
+#include <string.h>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#include <type_traits>
+
+using namespace qbf;
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...)
+{
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_err = buf;
+	return code;
+}
+
+#define HIPCHK(x)                                                                              \
+	do {                                                                                       \
+		hipError_t e_ = (x);                                                                   \
+		if (e_ != hipSuccess) return set_err(-2, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+	} while (0)
+
+extern "C" const char* qbf_last_error(void) { return g_err.c_str(); }
+
+// threads per workgroup for a padded P x P grid (P = 2N), and for an unpadded N x N grid
+template <typename R, int N> struct Cfg;
+template <> struct Cfg<double, 16> { static constexpr int T = 128; };
+template <> struct Cfg<double, 32> { static constexpr int T = 256; };
+template <> struct Cfg<float, 16> { static constexpr int T = 128; };
+template <> struct Cfg<float, 32> { static constexpr int T = 256; };
+template <> struct Cfg<float, 64> { static constexpr int T = 512; };
+template <int N> struct CfgNN;
+template <> struct CfgNN<16> { static constexpr int T = 64; };
+template <> struct CfgNN<32> { static constexpr int T = 128; };
+template <> struct CfgNN<64> { static constexpr int T = 256; };
+
+struct qbf_engine {
+	virtual ~qbf_engine() {}
+	int gridN = 0, precision = 0, maxCells = 0;
+	hipStream_t stream = nullptr;
+	bool ownStream = false;
+	virtual int set_i1(int slot, unsigned num, const double* p1, const double* p2) = 0;
+	virtual int set_i2(unsigned num, const double* p1, const double* p2) = 0;
+	virtual int set_i3(unsigned num, const double* p1, const double* p2) = 0;
+	virtual int collision(const double2* W, int ncell, int mode, double2* out, const double2* base,
+		const double2* Wcb, const double* Bext, double cf) = 0;
+	virtual int collision_step(const double2* W, int ncell, double dt, const double* Bext, double2* Wnext) = 0;
+	virtual int I1(int slot, const double2* f, const double2* g, const double2* h, double2* out) = 0;
+	virtual int I2(const double2* f, const double2* g, const double2* h, double2* out) = 0;
+	virtual int I3(const double2* f, const double2* g, const double2* h, double2* out) = 0;
+	virtual int phys2fourier(const double* in, int ngrid, double2* out) = 0;
+	virtual int fourier2phys(const double2* in, int ngrid, double* out) = 0;
+	virtual int transport(const double* W, int ncell, double L, double h, double dt, double* out) = 0;
+	virtual int inhom_step(const double* W, int ncell, double L, double h, double dt, const double* Bext, double* out) = 0;
+};
+
+namespace {
+
+struct DevBuf {
+	void* p = nullptr;
+	size_t bytes = 0;
+	int ensure(size_t n)
+	{
+		if (n <= bytes) return 0;
+		if (p) hipFree(p);
+		p = nullptr;
+		bytes = 0;
+		if (hipMalloc(&p, n) != hipSuccess) return set_err(-3, "hipMalloc(%zu) failed", n);
+		bytes = n;
+		return 0;
+	}
+	~DevBuf()
+	{
+		if (p) hipFree(p);
+	}
+	template <typename X> X* as() const { return (X*)p; }
+};
+
+double2* g_tw256 = nullptr;
+
+int ensure_twiddles()
+{
+	if (g_tw256) return 0;
+	double2 h[256];
+	for (int j = 0; j < 256; j++) h[j] = make_double2(cos(2.0 * M_PI * j / 256.0), sin(2.0 * M_PI * j / 256.0));
+	HIPCHK(hipMalloc(&g_tw256, sizeof(h)));
+	HIPCHK(hipMemcpy(g_tw256, h, sizeof(h), hipMemcpyHostToDevice));
+	return 0;
+}
+
+template <typename R, int N>
+struct EngineT final : public qbf_engine {
+	static constexpr int P = 2 * N, NN = N * N, PP = P * P;
+	static constexpr int T = Cfg<R, N>::T;
+	static constexpr int TN = CfgNN<N>::T;
+	using GE = Geo<P, T>;
+	using C = cx<R>;
+	static constexpr size_t GB = sizeof(C) * PP;    // bytes of one padded grid
+
+	struct I1Tab { int n = 0; DevBuf phi, psi1n; };
+	I1Tab i1[2];
+	int n2 = 0; DevBuf i2p1, i2p2;
+	int n3 = 0; DevBuf i3p1, i3p2n;
+
+	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, terms;
+	DevBuf Wmid, WF, WF2, Wt1, Wt2, src3;
+	size_t fBudget = (size_t)2 << 30;
+
+	int init()
+	{
+		int r = ensure_twiddles();
+		if (r) return r;
+		r = terms.ensure(sizeof(CTerm) * 4 * 16);
+		return r;
+	}
+
+	// --------------------------------------------------------------- tables
+	static void to_Ci_native(const double* tab, R* out)
+	{
+		for (int t = 0; t < T; t++) {
+			const int line = t / GE::G, p = t % GE::G;
+			for (int s = 0; s < GE::E; s++) {
+				const int l = p + GE::G * s, k = line;
+				out[(size_t)s * T + t] = (R)tab[k + (size_t)P * l];
+			}
+		}
+	}
+
+	int upload(DevBuf& b, const void* h, size_t bytes)
+	{
+		int r = b.ensure(bytes);
+		if (r) return r;
+		HIPCHK(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+		return 0;
+	}
+
+	int set_i1(int slot, unsigned num, const double* p1, const double* p2) override
+	{
+		if (slot < 0 || slot > 1) return set_err(-1, "bad I1 table slot %d", slot);
+		// group terms whose psiR1 grids are identical: sum_{q in g} psi1 .* (f (*) (psiR2_q g))
+		// = psi1 .* (f (*) ((sum_q psiR2_q) g)) exactly in exact arithmetic.
+		std::vector<int> rep;                    // representative term of each group
+		std::vector<std::vector<int>> members;
+		std::unordered_map<uint64_t, std::vector<int>> byHash;
+		for (unsigned q = 0; q < num; q++) {
+			const double* g = p1 + (size_t)q * PP;
+			uint64_t hsh = 1469598103934665603ULL;
+			const uint64_t* w = (const uint64_t*)g;
+			for (int i = 0; i < PP; i++) hsh = (hsh ^ w[i]) * 1099511628211ULL;
+			int found = -1;
+			auto it = byHash.find(hsh);
+			if (it != byHash.end())
+				for (int gi : it->second)
+					if (memcmp(p1 + (size_t)rep[gi] * PP, g, sizeof(double) * PP) == 0) {
+						found = gi;
+						break;
+					}
+			if (found < 0) {
+				found = (int)rep.size();
+				rep.push_back((int)q);
+				members.push_back({});
+				byHash[hsh].push_back(found);
+			}
+			members[found].push_back((int)q);
+		}
+		const int ng = (int)rep.size();
+		std::vector<double> phi((size_t)ng * NN, 0.0);
+		std::vector<R> psi1n((size_t)ng * PP);
+		for (int gi = 0; gi < ng; gi++) {
+			double* d = &phi[(size_t)gi * NN];
+			for (int q : members[gi])
+				for (int e = 0; e < NN; e++) d[e] += p2[(size_t)q * NN + e];
+			to_Ci_native(p1 + (size_t)rep[gi] * PP, &psi1n[(size_t)gi * PP]);
+		}
+		int r = upload(i1[slot].phi, phi.data(), sizeof(double) * phi.size());
+		if (!r) r = upload(i1[slot].psi1n, psi1n.data(), sizeof(R) * psi1n.size());
+		if (r) return r;
+		i1[slot].n = ng;
+		return 0;
+	}
+
+	int set_i2(unsigned num, const double* p1, const double* p2) override
+	{
+		int r = upload(i2p1, p1, sizeof(double) * NN * (size_t)num);
+		if (!r) r = upload(i2p2, p2, sizeof(double) * NN * (size_t)num);
+		if (r) return r;
+		n2 = (int)num;
+		return 0;
+	}
+
+	int set_i3(unsigned num, const double* p1, const double* p2) override
+	{
+		std::vector<R> p2n((size_t)num * PP);
+		for (unsigned q = 0; q < num; q++) to_Ci_native(p2 + (size_t)q * PP, &p2n[(size_t)q * PP]);
+		int r = upload(i3p1, p1, sizeof(double2) * NN * (size_t)num);
+		if (!r) r = upload(i3p2n, p2n.data(), sizeof(R) * p2n.size());
+		if (r) return r;
+		n3 = (int)num;
+		return 0;
+	}
+
+	// --------------------------------------------------------------- launches
+	int launch_padfft(const double2* src, int nsrc, const void* pre, int preKind, int q0, int nq, int nslot,
+		const int* srcOfSlot, void* out, int ncell)
+	{
+		PadFFTArgs a{};
+		a.src = src; a.nsrc = nsrc; a.pre = pre; a.preKind = preKind; a.q0 = q0; a.nq = nq; a.nslot = nslot;
+		for (int i = 0; i < nslot; i++) a.srcOfSlot[i] = srcOfSlot[i];
+		a.out = out; a.tw256 = g_tw256;
+		const int nb = ncell * nq * nslot;
+		hipLaunchKernelGGL((k_padfft<R, N, T>), dim3(nb), dim3(T), 0, stream, a);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	int launch_i1(int slot, const double2* src, int nsrc, const void* Fhat, int nFh, double fa, double fb,
+		int ninst, const int* instSrc, const int* instFh, void* out, int ncell)
+	{
+		if (i1[slot].n <= 0) return set_err(-1, "I1-type table slot %d not set", slot);
+		I1Args a{};
+		a.src = src; a.nsrc = nsrc; a.Fhat = Fhat; a.nFh = nFh; a.fa = fa; a.fb = fb;
+		a.phi = i1[slot].phi.template as<double>(); a.psi1n = i1[slot].psi1n.p; a.nterm = i1[slot].n;
+		a.ninst = ninst;
+		for (int i = 0; i < ninst; i++) { a.instSrc[i] = instSrc[i]; a.instFh[i] = instFh[i]; }
+		a.out = out; a.tw256 = g_tw256;
+		hipLaunchKernelGGL((k_i1acc<R, N, T>), dim3(ncell * ninst), dim3(T), 0, stream, a);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	int launch_i2(const double2* src, int nsrc, int ninst, const int* instF, const int* instG, void* out, int ncell)
+	{
+		if (n2 <= 0) return set_err(-1, "I2 table not set");
+		I2Args a{};
+		a.src = src; a.nsrc = nsrc; a.psi1 = i2p1.template as<double>(); a.psi2 = i2p2.template as<double>();
+		a.nterm = n2; a.ninst = ninst;
+		for (int i = 0; i < ninst; i++) { a.instF[i] = instF[i]; a.instG[i] = instG[i]; }
+		a.out = out; a.tw256 = g_tw256;
+		hipLaunchKernelGGL((k_i2acc<R, N, T>), dim3(ncell * ninst), dim3(T), 0, stream, a);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	// I3 over all quadrature terms; returns the number of accumulator chunks per (cell, out)
+	int run_i3(const double2* src, int nsrc, int nF, const int* srcOfF, const void* Shat, int nS, int nout,
+		const int* ninst, const int (*iF)[4], const int (*iS)[4], const int (*iH)[4], const double (*iW)[4],
+		int ncell, int* nchunkOut)
+	{
+		if (n3 <= 0) return set_err(-1, "I3 table not set");
+		const int nq = n3;
+		int nchunk = (1024 + ncell * nout - 1) / (ncell * nout);
+		if (nchunk > nq) nchunk = nq;
+		if (nchunk < 1) nchunk = 1;
+		const size_t perQ = (size_t)ncell * nF * GB;
+		int qr = (int)(fBudget / perQ);
+		if (qr < 1) qr = 1;
+		if (qr > nq) qr = nq;
+		if (nchunk > qr) nchunk = qr;
+		int r = Fscr.ensure(perQ * qr);
+		if (!r) r = acc3.ensure((size_t)ncell * nout * nchunk * GB);
+		if (r) return r;
+		for (int q0 = 0; q0 < nq; q0 += qr) {
+			const int nqr = (nq - q0) < qr ? (nq - q0) : qr;
+			r = launch_padfft(src, nsrc, i3p1.p, 2, q0, nqr, nF, srcOfF, Fscr.p, ncell);
+			if (r) return r;
+			I3Args a{};
+			a.Shat = Shat; a.nS = nS; a.F = Fscr.p; a.nF = nF; a.nq = nqr; a.q0 = q0;
+			a.psi2n = i3p2n.p; a.nout = nout;
+			for (int o = 0; o < nout; o++) {
+				a.ninst[o] = ninst[o];
+				for (int i = 0; i < ninst[o]; i++) {
+					a.instF[o][i] = iF[o][i]; a.instS[o][i] = iS[o][i]; a.instH[o][i] = iH[o][i]; a.instW[o][i] = iW[o][i];
+				}
+			}
+			a.nchunk = nchunk; a.accumulate = q0 > 0; a.acc = acc3.p; a.tw256 = g_tw256;
+			hipLaunchKernelGGL((k_i3acc<R, N, T>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			HIPCHK(hipGetLastError());
+		}
+		*nchunkOut = nchunk;
+		return 0;
+	}
+
+	int launch_combine(std::vector<CTerm> (&tl)[4], int nout, const int* outComp, int ncell, double2* out,
+		const double2* base, const double2* Wcb, const double* Bext, double cf)
+	{
+		int maxT = 1;
+		for (int o = 0; o < nout; o++) maxT = (int)tl[o].size() > maxT ? (int)tl[o].size() : maxT;
+		std::vector<CTerm> flat((size_t)nout * maxT);
+		for (int o = 0; o < nout; o++)
+			for (size_t i = 0; i < tl[o].size(); i++) flat[(size_t)o * maxT + i] = tl[o][i];
+		int r = terms.ensure(sizeof(CTerm) * flat.size());
+		if (r) return r;
+		HIPCHK(hipMemcpyAsync(terms.p, flat.data(), sizeof(CTerm) * flat.size(), hipMemcpyHostToDevice, stream));
+		CombineArgs a{};
+		a.terms = terms.template as<CTerm>(); a.maxTerms = maxT; a.nout = nout;
+		for (int o = 0; o < nout; o++) { a.nterm[o] = (int)tl[o].size(); a.outComp[o] = outComp[o]; }
+		a.outScale = 1.0; a.out = out; a.base = base; a.Wcb = Bext ? Wcb : nullptr; a.Bext = Bext; a.cf = cf;
+		a.tw256 = g_tw256;
+		hipLaunchKernelGGL((k_combine<R, N, T>), dim3(ncell * nout), dim3(T), 0, stream, a);
+		HIPCHK(hipGetLastError());
+		// the host term table must outlive the asynchronous copy
+		HIPCHK(hipStreamSynchronize(stream));
+		return 0;
+	}
+
+	static CTerm term(const void* X, long long xcell, const void* Y, long long ycell, double coef,
+		double ax = 1, double bx = 0, int nsum = 1, long long ysum = 0)
+	{
+		CTerm t{};
+		t.X = X; t.xcell = xcell; t.Y = Y; t.ycell = ycell; t.coef = coef; t.ax = ax; t.bx = bx;
+		t.nsum = nsum; t.ysum = ysum;
+		return t;
+	}
+
+	// --------------------------------------------------------------- collision operator
+	int collision(const double2* W, int ncell, int mode, double2* out, const double2* base,
+		const double2* Wcb, const double* Bext, double cf) override
+	{
+		if (ncell < 1 || ncell > maxCells) return set_err(-1, "ncell %d outside [1, %d]", ncell, maxCells);
+		const bool cc = mode & QBF_CC, cd = mode & QBF_CD;
+		const long long c4 = 4LL * PP;
+		int r = What.ensure((size_t)ncell * 4 * GB);
+		if (!r && cc) r = Ahat.ensure((size_t)ncell * 3 * GB);
+		if (!r && cd) r = Bhat.ensure((size_t)ncell * 4 * GB);
+		if (!r && cd) r = Sbuf.ensure((size_t)ncell * 4 * GB);
+		if (r) return r;
+		const int all4[4] = {0, 1, 2, 3}, zero4[4] = {0, 0, 0, 0};
+		r = launch_padfft(W, 4, nullptr, 0, 0, 1, 4, all4, What.p, ncell);
+		if (r) return r;
+		if (cc) {
+			// -2 (2 w_tr - 1) ... : Fhat = FFT2(pad(-4 W0 + 2 delta)) = -4 What_0 + 2 (src/collision.c:220-227)
+			const int g3[3] = {1, 2, 3};
+			r = launch_i1(QBF_TAB_I1, W, 4, What.p, 4, -4.0, 2.0, 3, g3, zero4, Ahat.p, ncell);
+			if (r) return r;
+		}
+		int nchunk = 1;
+		if (cd) {
+			// I4 with Fhat = FFT2(pad(2 W0 - delta)) = 2 What_0 - 1 (src/collision.c:127-134)
+			r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat.p, ncell);
+			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf.p, ncell);
+			if (r) return r;
+			static const double eta[4] = {1, -1, -1, -1};
+			int ni[4] = {4, 4, 4, 4};
+			int iF[4][4], iS[4][4], iH[4][4];
+			double iW[4][4];
+			for (int j = 0; j < 4; j++)
+				for (int i = 0; i < 4; i++) { iF[j][i] = i; iS[j][i] = i; iH[j][i] = j; iW[j][i] = eta[i]; }
+			r = run_i3(W, 4, 4, all4, What.p, 4, 4, ni, iF, iS, iH, iW, ncell, &nchunk);
+			if (r) return r;
+		}
+		const double p2 = 1.0 / ((double)P * P), p4 = p2 * p2;
+		const double eta2[4] = {2, -2, -2, -2};
+		const C* Wh = What.template as<C>();
+		auto Wslot = [&](int c) { return (const void*)(Wh + (size_t)c * PP); };
+		std::vector<CTerm> tl[4];
+		if (cc) {
+			const C* A = Ahat.template as<C>();
+			const long long a3 = 3LL * PP;
+			auto As = [&](int s) { return (const void*)(A + (size_t)s * PP); };
+			// Cc_1 = I1(f,W2,W3) - I1(f,W3,W2), cyclic (src/collision.c:231-253); Ahat slot s <-> g = s+1
+			tl[1].push_back(term(Wslot(3), c4, As(1), a3, p4));
+			tl[1].push_back(term(Wslot(2), c4, As(2), a3, -p4));
+			tl[2].push_back(term(Wslot(1), c4, As(2), a3, p4));
+			tl[2].push_back(term(Wslot(3), c4, As(0), a3, -p4));
+			tl[3].push_back(term(Wslot(2), c4, As(0), a3, p4));
+			tl[3].push_back(term(Wslot(1), c4, As(1), a3, -p4));
+		}
+		if (cd) {
+			const C* B = Bhat.template as<C>();
+			const C* S = Sbuf.template as<C>();
+			const C* A3 = acc3.template as<C>();
+			auto Bs = [&](int s) { return (const void*)(B + (size_t)s * PP); };
+			for (int j = 0; j < 4; j++) {
+				// I4 terms (src/collision.c:137-167)
+				tl[j].push_back(term(Wslot(0), c4, Bs(j), c4, eta2[j] * p4));
+				tl[j].push_back(term(Wslot(j), c4, Bs(0), c4, 2 * p4));
+				if (j == 0)
+					for (int jj = 0; jj < 4; jj++) tl[j].push_back(term(Wslot(jj), c4, Bs(jj), c4, -2 * p4));
+				// I3 terms: -eta2_j / P^4 * sum_chunks acc_j (src/collision.c:110-121)
+				tl[j].push_back(term(nullptr, 0, A3 + (size_t)j * nchunk * PP, 4LL * nchunk * PP, -eta2[j] * p4,
+					1, 0, nchunk, PP));
+				// I2 terms: eta2_i / P^2 * (delta_j0 - What_j) .* S_i (src/collision.c:82-106)
+				for (int i = 0; i < 4; i++)
+					tl[j].push_back(term(Wslot(j), c4, S + (size_t)i * PP, c4, eta2[i] * p2, -1.0, j == 0 ? 1.0 : 0.0));
+			}
+		}
+		return launch_combine(tl, 4, all4, ncell, out, base, Wcb, Bext, cf);
+	}
+
+	int collision_step(const double2* W, int ncell, double dt, const double* Bext, double2* Wnext) override
+	{
+		int r = Wmid.ensure(sizeof(double2) * 4 * NN * (size_t)ncell);
+		if (r) return r;
+		double2* Wm = Wmid.template as<double2>();
+		r = collision(W, ncell, QBF_C, Wm, W, W, Bext, 0.5 * dt);
+		if (!r) r = collision(Wm, ncell, QBF_C, Wnext, W, Wm, Bext, dt);
+		return r;
+	}
+
+	// --------------------------------------------------------------- single integrals (one grid)
+	int stage3(const double2* f, const double2* g, const double2* h)
+	{
+		int r = src3.ensure(sizeof(double2) * 3 * NN);
+		if (r) return r;
+		double2* s = src3.template as<double2>();
+		HIPCHK(hipMemcpyAsync(s, f, sizeof(double2) * NN, hipMemcpyDeviceToDevice, stream));
+		HIPCHK(hipMemcpyAsync(s + NN, g, sizeof(double2) * NN, hipMemcpyDeviceToDevice, stream));
+		HIPCHK(hipMemcpyAsync(s + 2 * NN, h, sizeof(double2) * NN, hipMemcpyDeviceToDevice, stream));
+		return 0;
+	}
+
+	int I1(int slot, const double2* f, const double2* g, const double2* h, double2* out) override
+	{
+		int r = stage3(f, g, h);
+		if (!r) r = What.ensure(2 * GB);
+		if (!r) r = Ahat.ensure(GB);
+		if (r) return r;
+		const double2* s = src3.template as<double2>();
+		const int fh[2] = {0, 2}, one[1] = {1}, z[1] = {0};
+		r = launch_padfft(s, 3, nullptr, 0, 0, 1, 2, fh, What.p, 1);
+		if (!r) r = launch_i1(slot, s, 3, What.p, 2, 1.0, 0.0, 1, one, z, Ahat.p, 1);
+		if (r) return r;
+		const double p4 = 1.0 / ((double)P * P * P * P);
+		std::vector<CTerm> tl[4];
+		tl[0].push_back(term(What.template as<C>() + PP, 0, Ahat.p, 0, p4));
+		const int c0[1] = {0};
+		return launch_combine(tl, 1, c0, 1, out, nullptr, nullptr, nullptr, 1.0);
+	}
+
+	int I2(const double2* f, const double2* g, const double2* h, double2* out) override
+	{
+		int r = stage3(f, g, h);
+		if (!r) r = What.ensure(GB);
+		if (!r) r = Sbuf.ensure(GB);
+		if (r) return r;
+		const double2* s = src3.template as<double2>();
+		const int hh[1] = {2}, f0[1] = {0}, g1[1] = {1};
+		r = launch_padfft(s, 3, nullptr, 0, 0, 1, 1, hh, What.p, 1);
+		if (!r) r = launch_i2(s, 3, 1, f0, g1, Sbuf.p, 1);
+		if (r) return r;
+		const double p2 = 1.0 / ((double)P * P);
+		std::vector<CTerm> tl[4];
+		tl[0].push_back(term(What.p, 0, Sbuf.p, 0, p2));
+		const int c0[1] = {0};
+		return launch_combine(tl, 1, c0, 1, out, nullptr, nullptr, nullptr, 1.0);
+	}
+
+	int I3(const double2* f, const double2* g, const double2* h, double2* out) override
+	{
+		int r = stage3(f, g, h);
+		if (!r) r = What.ensure(GB);
+		if (r) return r;
+		const double2* s = src3.template as<double2>();
+		const int gg[1] = {1}, fh[2] = {0, 2};
+		r = launch_padfft(s, 3, nullptr, 0, 0, 1, 1, gg, What.p, 1);
+		if (r) return r;
+		int ni[4] = {1, 0, 0, 0};
+		int iF[4][4] = {{0}}, iS[4][4] = {{0}}, iH[4][4] = {{1}};
+		double iW[4][4] = {{1.0}};
+		int nchunk = 1;
+		r = run_i3(s, 3, 2, fh, What.p, 1, 1, ni, iF, iS, iH, iW, 1, &nchunk);
+		if (r) return r;
+		const double p4 = 1.0 / ((double)P * P * P * P);
+		std::vector<CTerm> tl[4];
+		tl[0].push_back(term(nullptr, 0, acc3.p, 0, p4, 1, 0, nchunk, PP));
+		const int c0[1] = {0};
+		return launch_combine(tl, 1, c0, 1, out, nullptr, nullptr, nullptr, 1.0);
+	}
+
+	// --------------------------------------------------------------- state transforms and transport
+	int phys2fourier(const double* in, int ngrid, double2* out) override
+	{
+		hipLaunchKernelGGL((k_phys2fourier<N, TN>), dim3(ngrid), dim3(TN), 0, stream, in, out, g_tw256, ngrid);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	int fourier2phys(const double2* in, int ngrid, double* out) override
+	{
+		hipLaunchKernelGGL((k_fourier2phys<N, TN>), dim3(ngrid), dim3(TN), 0, stream, in, out, g_tw256, ngrid);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	int transport(const double* W, int ncell, double L, double h, double dt, double* out) override
+	{
+		if (ncell < 1) return set_err(-1, "ncell must be >= 1");
+		const int chunk = ncell < 32 ? ncell : 32;
+		const long long nthr = 4LL * NN * ((ncell + chunk - 1) / chunk);
+		hipLaunchKernelGGL(k_transport_periodic, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, stream,
+			W, out, N, ncell, chunk, h, dt, L);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	int inhom_step(const double* W, int ncell, double L, double h, double dt, const double* Bext, double* out) override
+	{
+		if (ncell < 1 || ncell > maxCells) return set_err(-1, "ncell %d outside [1, %d]", ncell, maxCells);
+		const size_t nphys = sizeof(double) * 4 * NN * (size_t)ncell;
+		int r = Wt1.ensure(nphys);
+		if (!r) r = Wt2.ensure(nphys);
+		if (!r) r = WF.ensure(2 * nphys);
+		if (!r) r = WF2.ensure(2 * nphys);
+		if (r) return r;
+		r = transport(W, ncell, L, h, 0.5 * dt, Wt1.template as<double>());
+		if (!r) r = phys2fourier(Wt1.template as<double>(), 4 * ncell, WF.template as<double2>());
+		if (!r) r = collision_step(WF.template as<double2>(), ncell, dt, Bext, WF2.template as<double2>());
+		if (!r) r = fourier2phys(WF2.template as<double2>(), 4 * ncell, Wt2.template as<double>());
+		if (!r) r = transport(Wt2.template as<double>(), ncell, L, h, 0.5 * dt, out);
+		return r;
+	}
+};
+
+template <typename R, int N>
+int make_engine(int maxCells, qbf_engine** out)
+{
+	auto* e = new EngineT<R, N>();
+	e->gridN = N;
+	e->precision = std::is_same<R, float>::value ? QBF_FP32 : QBF_FP64;
+	e->maxCells = maxCells;
+	int r = e->init();
+	if (r) {
+		delete e;
+		return r;
+	}
+	*out = e;
+	return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int qbf_engine_create(int N, int precision, int maxCells, qbf_engine** out)
+{
+	if (!out) return set_err(-1, "null output pointer");
+	*out = nullptr;
+	if (maxCells < 1) return set_err(-1, "maxCells must be >= 1");
+	qbf_engine* e = nullptr;
+	int r;
+	if (precision == QBF_FP64 && N == 16) r = make_engine<double, 16>(maxCells, &e);
+	else if (precision == QBF_FP64 && N == 32) r = make_engine<double, 32>(maxCells, &e);
+	else if (precision == QBF_FP32 && N == 16) r = make_engine<float, 16>(maxCells, &e);
+	else if (precision == QBF_FP32 && N == 32) r = make_engine<float, 32>(maxCells, &e);
+	else if (precision == QBF_FP32 && N == 64) r = make_engine<float, 64>(maxCells, &e);
+	else return set_err(-1, "unsupported (N=%d, precision=%d)", N, precision);
+	if (r) return r;
+	hipStream_t s;
+	if (hipStreamCreate(&s) != hipSuccess) {
+		delete e;
+		return set_err(-2, "hipStreamCreate failed");
+	}
+	e->stream = s;
+	e->ownStream = true;
+	*out = e;
+	return 0;
+}
+
+void qbf_engine_destroy(qbf_engine* e)
+{
+	if (!e) return;
+	if (e->stream) hipStreamSynchronize(e->stream);
+	if (e->ownStream && e->stream) hipStreamDestroy(e->stream);
+	delete e;
+}
+
+int qbf_engine_N(const qbf_engine* e) { return e ? e->gridN : 0; }
+int qbf_engine_precision(const qbf_engine* e) { return e ? e->precision : -1; }
+
+int qbf_set_stream(qbf_engine* e, void* s)
+{
+	if (!e) return set_err(-1, "null engine");
+	if (e->ownStream && e->stream) {
+		hipStreamSynchronize(e->stream);
+		hipStreamDestroy(e->stream);
+	}
+	if (s) {
+		e->stream = (hipStream_t)s;
+		e->ownStream = false;
+	} else {
+		HIPCHK(hipStreamCreate(&e->stream));
+		e->ownStream = true;
+	}
+	return 0;
+}
+
+int qbf_sync(qbf_engine* e)
+{
+	if (!e) return set_err(-1, "null engine");
+	HIPCHK(hipStreamSynchronize(e->stream));
+	return 0;
+}
+
+int qbf_set_quadrature(qbf_engine* e, const qbf_quad_tables* q)
+{
+	if (!e || !q) return set_err(-1, "null argument");
+	int r = e->set_i1(QBF_TAB_I1, q->numI1, q->I1_psiR1, q->I1_psiR2);
+	if (!r) r = e->set_i1(QBF_TAB_I4, q->numI4, q->I4_psiR1, q->I4_psiR2);
+	if (!r) r = e->set_i2(q->numI2, q->I2_psiR1, q->I2_psiR2);
+	if (!r) r = e->set_i3(q->numI3, q->I3_psiR1, q->I3_psiR2);
+	return r;
+}
+
+int qbf_set_i1_table(qbf_engine* e, int slot, unsigned num, const double* p1, const double* p2)
+{
+	if (!e || !p1 || !p2 || num == 0) return set_err(-1, "bad argument");
+	return e->set_i1(slot, num, p1, p2);
+}
+int qbf_set_i2_table(qbf_engine* e, unsigned num, const double* p1, const double* p2)
+{
+	if (!e || !p1 || !p2 || num == 0) return set_err(-1, "bad argument");
+	return e->set_i2(num, p1, p2);
+}
+int qbf_set_i3_table(qbf_engine* e, unsigned num, const double* p1, const double* p2)
+{
+	if (!e || !p1 || !p2 || num == 0) return set_err(-1, "bad argument");
+	return e->set_i3(num, p1, p2);
+}
+
+int qbf_collision(qbf_engine* e, const double* dW, int ncell, int mode, double* dOut, const double* dBase,
+	const double* dWcb, const double* dBext, double cf)
+{
+	if (!e || !dW || !dOut) return set_err(-1, "null argument");
+	if (mode < 1 || mode > 3) return set_err(-1, "bad mode %d", mode);
+	return e->collision((const double2*)dW, ncell, mode, (double2*)dOut, (const double2*)dBase,
+		(const double2*)dWcb, dBext, cf);
+}
+
+int qbf_collision_step(qbf_engine* e, const double* dW, int ncell, double dt, const double* dBext, double* dWnext)
+{
+	if (!e || !dW || !dWnext) return set_err(-1, "null argument");
+	return e->collision_step((const double2*)dW, ncell, dt, dBext, (double2*)dWnext);
+}
+
+int qbf_I1(qbf_engine* e, int slot, const double* f, const double* g, const double* h, double* out)
+{
+	if (!e || !f || !g || !h || !out) return set_err(-1, "null argument");
+	if (slot != QBF_TAB_I1 && slot != QBF_TAB_I4) return set_err(-1, "bad slot");
+	return e->I1(slot, (const double2*)f, (const double2*)g, (const double2*)h, (double2*)out);
+}
+int qbf_I2(qbf_engine* e, const double* f, const double* g, const double* h, double* out)
+{
+	if (!e || !f || !g || !h || !out) return set_err(-1, "null argument");
+	return e->I2((const double2*)f, (const double2*)g, (const double2*)h, (double2*)out);
+}
+int qbf_I3(qbf_engine* e, const double* f, const double* g, const double* h, double* out)
+{
+	if (!e || !f || !g || !h || !out) return set_err(-1, "null argument");
+	return e->I3((const double2*)f, (const double2*)g, (const double2*)h, (double2*)out);
+}
+
+int qbf_phys_to_fourier(qbf_engine* e, const double* in, int ngrid, double* out)
+{
+	if (!e || !in || !out || ngrid < 1) return set_err(-1, "bad argument");
+	return e->phys2fourier(in, ngrid, (double2*)out);
+}
+int qbf_fourier_to_phys(qbf_engine* e, const double* in, int ngrid, double* out)
+{
+	if (!e || !in || !out || ngrid < 1) return set_err(-1, "bad argument");
+	return e->fourier2phys((const double2*)in, ngrid, out);
+}
+
+int qbf_transport_periodic(qbf_engine* e, const double* W, int ncell, double L, double h, double dt, double* out)
+{
+	if (!e || !W || !out) return set_err(-1, "null argument");
+	return e->transport(W, ncell, L, h, dt, out);
+}
+
+int qbf_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L, double h, double dt,
+	const double* Bext, double* out)
+{
+	if (!e || !W || !out) return set_err(-1, "null argument");
+	return e->inhom_step(W, ncell, L, h, dt, Bext, out);
+}
+
+void* qbf_malloc(size_t bytes)
+{
+	void* p = nullptr;
+	if (hipMalloc(&p, bytes) != hipSuccess) {
+		set_err(-3, "hipMalloc(%zu) failed", bytes);
+		return nullptr;
+	}
+	return p;
+}
+
+void qbf_free(void* p)
+{
+	if (p) hipFree(p);
+}
+
+int qbf_h2d(qbf_engine* e, void* dst, const void* src, size_t bytes)
+{
+	HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e ? e->stream : nullptr));
+	HIPCHK(hipStreamSynchronize(e ? e->stream : nullptr));
+	return 0;
+}
+
+int qbf_d2h(qbf_engine* e, void* dst, const void* src, size_t bytes)
+{
+	HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e ? e->stream : nullptr));
+	HIPCHK(hipStreamSynchronize(e ? e->stream : nullptr));
+	return 0;
+}
+
+} // extern "C"

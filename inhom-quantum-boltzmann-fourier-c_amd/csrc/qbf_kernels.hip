@@ -1,0 +1,428 @@
+// qbf_kernels.hip -- HIP kernels of the collision operator and the transport step (gfx950).
+//
+// All collision kernels work on zero-padded P x P grids (P = 2N) held by one
+// workgroup (see qbf_device.hpp for the register layouts).  The collision
+// operator C = Cc + Cd of the reference (src/collision.c:74-168, 209-254) is
+// evaluated in an exactly reassociated form: every term that the reference
+// convolves, pads, extracts and sums separately is summed in the transformed
+// domain instead, so each Cc+Cd evaluation needs 15 + 22J + 36JM padded FFTs
+// instead of the reference's 68 + 96J + 24J^2 + 80JM.
+#include "qbf_device.hpp"
+#include "qbf_kernels.hpp"
+
+namespace qbf {
+
+template <int N>
+__device__ __forceinline__ int unpad(int x)
+{
+	// padded frequency index (axis [0..N-1, -N..-1]) -> index of the N x N grid
+	// (axis [0..N/2-1, -N/2..-1]) or -1 for a padding zero (src/grid.c:171-188)
+	return x < N / 2 ? x : (x >= 2 * N - N / 2 ? x - N : -1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// pruned FFT2 of pad(pre .* src): C_i load -> column FFT -> LDS -> row FFT -> R_b (registers)
+template <typename R, int N, class GE>
+__device__ __forceinline__ void padfft_reg(cx<R>* v, const double2* __restrict__ src, const void* pre, int preKind,
+	const Xpose<R, GE>& X, const cx<R>* tw, int line, int p)
+{
+	constexpr int G = GE::G, E = GE::E, P = GE::P;
+	const int kn = unpad<N>(line);
+	if (kn >= 0) {
+#pragma unroll
+		for (int s = 0; s < E; s++) {
+			const int ln = unpad<N>(p + G * s);
+			if (ln >= 0) {
+				const int e = ln * N + kn;
+				double2 x = src[e];
+				if (preKind == 1) {
+					const double r = ((const double*)pre)[e];
+					x.x *= r;
+					x.y *= r;
+				} else if (preKind == 2) {
+					const double2 c = ((const double2*)pre)[e];
+					x = make_double2(c.x * x.x - c.y * x.y, c.x * x.y + c.y * x.x);
+				}
+				v[s] = mk<R>((R)x.x, (R)x.y);
+			} else {
+				v[s] = czero<R>();
+			}
+		}
+		line_fwd<R, P, G, E>(v, p, tw);
+	} else {
+#pragma unroll
+		for (int s = 0; s < E; s++) v[s] = czero<R>();
+	}
+	__syncthreads();
+	X.put_Cb(v, line, p);
+	__syncthreads();
+	X.get_Ri(v, line, p);
+	line_fwd<R, P, G, E>(v, p, tw);
+}
+
+// R_b (physical) -> IFFT2 -> C_i (frequency)
+template <typename R, class GE>
+__device__ __forceinline__ void ifft2_reg(cx<R>* v, const Xpose<R, GE>& X, const cx<R>* tw, int line, int p)
+{
+	constexpr int G = GE::G, E = GE::E, P = GE::P;
+	line_bwd<R, P, G, E>(v, p, tw);
+	__syncthreads();
+	X.put_Ri(v, line, p);
+	__syncthreads();
+	X.get_Cb(v, line, p);
+	line_bwd<R, P, G, E>(v, p, tw);
+}
+
+// C_i (frequency) -> FFT2 -> R_b (physical)
+template <typename R, class GE>
+__device__ __forceinline__ void fft2_reg(cx<R>* v, const Xpose<R, GE>& X, const cx<R>* tw, int line, int p)
+{
+	constexpr int G = GE::G, E = GE::E, P = GE::P;
+	line_fwd<R, P, G, E>(v, p, tw);
+	__syncthreads();
+	X.put_Cb(v, line, p);
+	__syncthreads();
+	X.get_Ri(v, line, p);
+	line_fwd<R, P, G, E>(v, p, tw);
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_padfft(PadFFTArgs a)
+{
+	using GE = Geo<2 * N, T>;
+	constexpr int P = GE::P, E = GE::E;
+	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> tw[P];
+	load_twiddles<R, P, T>(tw, a.tw256);
+	const int o = blockIdx.x;
+	const int slot = o % a.nslot;
+	const int qq = (o / a.nslot) % a.nq;
+	const int cell = o / (a.nslot * a.nq);
+	const double2* src = a.src + ((size_t)cell * a.nsrc + a.srcOfSlot[slot]) * (N * N);
+	const void* pre = nullptr;
+	if (a.preKind == 1) pre = (const double*)a.pre + (size_t)(a.q0 + qq) * (N * N);
+	if (a.preKind == 2) pre = (const double2*)a.pre + (size_t)(a.q0 + qq) * (N * N);
+	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
+	Xpose<R, GE> X{lds};
+	cx<R> v[E];
+	__syncthreads();
+	padfft_reg<R, N, GE>(v, src, pre, a.preKind, X, tw, line, p);
+	store_native<R, GE>((cx<R>*)a.out + (size_t)o * (P * P), v);
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_i1acc(I1Args a)
+{
+	using GE = Geo<2 * N, T>;
+	constexpr int P = GE::P, E = GE::E;
+	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> tw[P];
+	load_twiddles<R, P, T>(tw, a.tw256);
+	const int inst = blockIdx.x % a.ninst;
+	const int cell = blockIdx.x / a.ninst;
+	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
+	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instSrc[inst]) * (N * N);
+	const cx<R>* Fh = (const cx<R>*)a.Fhat + ((size_t)cell * a.nFh + a.instFh[inst]) * (P * P);
+	const R fa = (R)a.fa, fb = (R)a.fb;
+	Xpose<R, GE> X{lds};
+	cx<R> acc[E], v[E];
+#pragma unroll
+	for (int s = 0; s < E; s++) acc[s] = czero<R>();
+	__syncthreads();
+	for (int q = 0; q < a.nterm; q++) {
+		padfft_reg<R, N, GE>(v, g, a.phi + (size_t)q * (N * N), 1, X, tw, line, p);
+#pragma unroll
+		for (int u = 0; u < E; u++) {
+			cx<R> f = Fh[u * T + threadIdx.x];
+			f = mk<R>(fa * f.x + fb, fa * f.y);
+			v[u] = cmul<R>(f, v[u]);
+		}
+		ifft2_reg<R, GE>(v, X, tw, line, p);
+		const R* ps = (const R*)a.psi1n + (size_t)q * (P * P);
+#pragma unroll
+		for (int s = 0; s < E; s++) {
+			const R w = ps[s * T + threadIdx.x];
+			acc[s].x += w * v[s].x;
+			acc[s].y += w * v[s].y;
+		}
+	}
+	fft2_reg<R, GE>(acc, X, tw, line, p);
+	store_native<R, GE>((cx<R>*)a.out + ((size_t)cell * a.ninst + inst) * (P * P), acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_i2acc(I2Args a)
+{
+	using GE = Geo<2 * N, T>;
+	constexpr int P = GE::P, E = GE::E;
+	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> tw[P];
+	load_twiddles<R, P, T>(tw, a.tw256);
+	const int inst = blockIdx.x % a.ninst;
+	const int cell = blockIdx.x / a.ninst;
+	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
+	const double2* f = a.src + ((size_t)cell * a.nsrc + a.instF[inst]) * (N * N);
+	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instG[inst]) * (N * N);
+	Xpose<R, GE> X{lds};
+	cx<R> acc[E], x1[E], v[E];
+#pragma unroll
+	for (int u = 0; u < E; u++) acc[u] = czero<R>();
+	__syncthreads();
+	for (int q = 0; q < a.nterm; q++) {
+		padfft_reg<R, N, GE>(x1, f, a.psi1 + (size_t)q * (N * N), 1, X, tw, line, p);
+		padfft_reg<R, N, GE>(v, g, a.psi2 + (size_t)q * (N * N), 1, X, tw, line, p);
+#pragma unroll
+		for (int u = 0; u < E; u++) {
+			const cx<R> t = cmul<R>(x1[u], v[u]);
+			acc[u] = cadd<R>(acc[u], t);
+		}
+	}
+	store_native<R, GE>((cx<R>*)a.out + ((size_t)cell * a.ninst + inst) * (P * P), acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The dominant kernel: 16 applications of K_q per quadrature term and cell for CdInt.
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_i3acc(I3Args a)
+{
+	using GE = Geo<2 * N, T>;
+	constexpr int P = GE::P, E = GE::E;
+	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> tw[P];
+	load_twiddles<R, P, T>(tw, a.tw256);
+	const int chunk = blockIdx.x % a.nchunk;
+	const int out = (blockIdx.x / a.nchunk) % a.nout;
+	const int cell = blockIdx.x / (a.nchunk * a.nout);
+	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
+	const int t = threadIdx.x;
+	const int qa = (int)(((long long)chunk * a.nq) / a.nchunk);
+	const int qb = (int)(((long long)(chunk + 1) * a.nq) / a.nchunk);
+	cx<R>* accp = (cx<R>*)a.acc + (((size_t)cell * a.nout + out) * a.nchunk + chunk) * (P * P);
+	Xpose<R, GE> X{lds};
+	cx<R> acc[E], v[E];
+	if (a.accumulate) {
+		load_native<R, GE>(acc, accp);
+	} else {
+#pragma unroll
+		for (int u = 0; u < E; u++) acc[u] = czero<R>();
+	}
+	const cx<R>* Sh = (const cx<R>*)a.Shat + (size_t)cell * a.nS * (P * P);
+	const int ni = a.ninst[out];
+	__syncthreads();
+	for (int qq = qa; qq < qb; qq++) {
+		const cx<R>* Fq = (const cx<R>*)a.F + ((size_t)cell * a.nq + qq) * a.nF * (P * P);
+		const R* ps = (const R*)a.psi2n + (size_t)(a.q0 + qq) * (P * P);
+		for (int in = 0; in < ni; in++) {
+			const cx<R>* Sg = Sh + (size_t)a.instS[out][in] * (P * P);
+			const cx<R>* Fh = Fq + (size_t)a.instH[out][in] * (P * P);
+			const cx<R>* Ff = Fq + (size_t)a.instF[out][in] * (P * P);
+			const R w = (R)a.instW[out][in];
+#pragma unroll
+			for (int u = 0; u < E; u++) v[u] = cmul<R>(Sg[u * T + t], Fh[u * T + t]);
+			ifft2_reg<R, GE>(v, X, tw, line, p);
+#pragma unroll
+			for (int s = 0; s < E; s++) {
+				const R m = ps[s * T + t];
+				v[s].x *= m;
+				v[s].y *= m;
+			}
+			fft2_reg<R, GE>(v, X, tw, line, p);
+#pragma unroll
+			for (int u = 0; u < E; u++) {
+				const cx<R> y = cmul<R>(Ff[u * T + t], v[u]);
+				acc[u].x += w * y.x;
+				acc[u].y += w * y.y;
+			}
+		}
+	}
+	store_native<R, GE>(accp, acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_combine(CombineArgs a)
+{
+	using GE = Geo<2 * N, T>;
+	constexpr int P = GE::P, E = GE::E, G = GE::G;
+	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> tw[P];
+	load_twiddles<R, P, T>(tw, a.tw256);
+	const int o = blockIdx.x % a.nout;
+	const int cell = blockIdx.x / a.nout;
+	const int line = threadIdx.x / G, p = threadIdx.x % G;
+	const int t = threadIdx.x;
+	Xpose<R, GE> X{lds};
+	cx<R> v[E];
+#pragma unroll
+	for (int u = 0; u < E; u++) v[u] = czero<R>();
+	for (int it = 0; it < a.nterm[o]; it++) {
+		const CTerm c = a.terms[o * a.maxTerms + it];
+		const cx<R>* Xp = (const cx<R>*)c.X + (size_t)cell * c.xcell;
+		const cx<R>* Yp = (const cx<R>*)c.Y + (size_t)cell * c.ycell;
+		const R ax = (R)c.ax, bx = (R)c.bx, cf = (R)c.coef;
+#pragma unroll
+		for (int u = 0; u < E; u++) {
+			cx<R> y = mk<R>(R(1), R(0));
+			if (c.Y) {
+				y = Yp[u * T + t];
+				for (int n = 1; n < c.nsum; n++) y = cadd<R>(y, Yp[(size_t)n * c.ysum + u * T + t]);
+			}
+			if (c.X) {
+				cx<R> x = Xp[u * T + t];
+				x = mk<R>(ax * x.x + bx, ax * x.y);
+				y = cmul<R>(x, y);
+			}
+			v[u].x += cf * y.x;
+			v[u].y += cf * y.y;
+		}
+	}
+	__syncthreads();
+	ifft2_reg<R, GE>(v, X, tw, line, p);
+	// C_i: this thread holds (l = p + G*s, k = line); keep the N x N block (src/grid.c:195-210)
+	const int kn = unpad<N>(line);
+	if (kn < 0) return;
+	const int comp = a.outComp[o];
+	const size_t cb = (size_t)cell * 4 * (N * N);
+#pragma unroll
+	for (int s = 0; s < E; s++) {
+		const int ln = unpad<N>(p + G * s);
+		if (ln < 0) continue;
+		const int e = ln * N + kn;
+		double vr = a.outScale * (double)v[s].x, vi = a.outScale * (double)v[s].y;
+		if (a.Wcb && comp > 0) {
+			// magnetic rotation 2 (B x w) (src/simulation.c:74-98)
+			const double* B = a.Bext + 3 * cell;
+			const int c1 = comp == 1 ? 2 : (comp == 2 ? 3 : 1);
+			const int c2 = comp == 1 ? 3 : (comp == 2 ? 1 : 2);
+			const int b1 = comp == 1 ? 1 : (comp == 2 ? 2 : 0);
+			const int b2 = comp == 1 ? 2 : (comp == 2 ? 0 : 1);
+			// comp 1: 2(B1 w3 - B2 w2); comp 2: 2(B2 w1 - B0 w3); comp 3: 2(B0 w2 - B1 w1)
+			const double2 wa = a.Wcb[cb + (size_t)c2 * (N * N) + e];
+			const double2 wb = a.Wcb[cb + (size_t)c1 * (N * N) + e];
+			vr += 2 * (B[b1] * wa.x - B[b2] * wb.x);
+			vi += 2 * (B[b1] * wa.y - B[b2] * wb.y);
+		}
+		double2 r = make_double2(a.cf * vr, a.cf * vi);
+		if (a.base) {
+			const double2 b = a.base[cb + (size_t)comp * (N * N) + e];
+			r = make_double2(b.x + r.x, b.y + r.y);
+		}
+		a.out[cb + (size_t)comp * (N * N) + e] = r;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// N x N transforms of the state (src/simulation.c:224-283): physical real -> Fourier / N^2
+// (full complex FFT of the real data), and Fourier -> unnormalised backward FFT -> real part.
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_phys2fourier(const double* __restrict__ in, double2* __restrict__ out,
+	const double2* tw256, int ngrid)
+{
+	using GE = Geo<N, T>;
+	constexpr int E = GE::E, G = GE::G;
+	__shared__ cx<double> lds[N * GE::PITCH];
+	__shared__ cx<double> tw[N];
+	load_twiddles<double, N, T>(tw, tw256);
+	const int gi = blockIdx.x;
+	if (gi >= ngrid) return;
+	const int line = threadIdx.x / G, p = threadIdx.x % G;
+	const double* src = in + (size_t)gi * (N * N);
+	Xpose<double, GE> X{lds};
+	cx<double> v[E];
+#pragma unroll
+	for (int s = 0; s < E; s++) v[s] = make_double2(src[(p + G * s) * N + line], 0.0);
+	__syncthreads();
+	fft2_reg<double, GE>(v, X, tw, line, p);
+	const double sc = 1.0 / ((double)N * N);
+	double2* dst = out + (size_t)gi * (N * N);
+	const int k0 = E * brev<G>(p);
+#pragma unroll
+	for (int u = 0; u < E; u++) dst[line * N + k0 + u] = make_double2(v[u].x * sc, v[u].y * sc);
+}
+
+template <int N, int T>
+__global__ __launch_bounds__(T) void k_fourier2phys(const double2* __restrict__ in, double* __restrict__ out,
+	const double2* tw256, int ngrid)
+{
+	using GE = Geo<N, T>;
+	constexpr int E = GE::E, G = GE::G;
+	__shared__ cx<double> lds[N * GE::PITCH];
+	__shared__ cx<double> tw[N];
+	load_twiddles<double, N, T>(tw, tw256);
+	const int gi = blockIdx.x;
+	if (gi >= ngrid) return;
+	const int line = threadIdx.x / G, p = threadIdx.x % G;
+	const double2* src = in + (size_t)gi * (N * N);
+	Xpose<double, GE> X{lds};
+	cx<double> v[E];
+	const int k0 = E * brev<G>(p);
+#pragma unroll
+	for (int u = 0; u < E; u++) v[u] = src[line * N + k0 + u];
+	__syncthreads();
+	ifft2_reg<double, GE>(v, X, tw, line, p);
+	double* dst = out + (size_t)gi * (N * N);
+#pragma unroll
+	for (int s = 0; s < E; s++) dst[(p + G * s) * N + line] = v[s].x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Slope-limited upwind transport along x with periodic boundaries (src/finite_volume.c:35-104),
+// applied to every (component, l, k) line of a [ncell][4][N][N] state.  Velocity A = vgrid[l].
+// One thread per (line element, chunk of cells); the arithmetic follows the reference's
+// expression order with contraction disabled, so results are bit-identical to the C code.
+__device__ __forceinline__ double minmod_d(double a, double b)
+{
+	return 0.5 * (copysign(1.0, a) + copysign(1.0, b)) * fmin(fabs(a), fabs(b));
+}
+
+__global__ __launch_bounds__(256) void k_transport_periodic(const double* __restrict__ Un, double* __restrict__ Un1,
+	int N, int ncell, int chunk, double h, double dt, double L)
+{
+#pragma clang fp contract(off)
+	const int NN = N * N;
+	const long long nel = 4LL * NN;
+	const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int nchunks = (ncell + chunk - 1) / chunk;
+	if (gid >= nel * nchunks) return;
+	const int e = (int)(gid % nel);          // component * N*N + l*N + k
+	const int ch = (int)(gid / nel);
+	const int l = (e % NN) / N;
+	const double scl = (2 * L) / N;
+	const double A = l < N / 2 ? scl * l : scl * (l - N);
+	const double Aabs = fabs(A);
+	const double nu = dt * A / h;
+	const double S = 0.5 * A * (copysign(1.0, nu) - nu);
+	const double Sabs = 0.5 * Aabs * (copysign(1.0, nu) - nu);
+	const double scale = dt / (2 * h);
+	const int c0 = ch * chunk;
+	const int c1 = min(ncell, c0 + chunk);
+	auto U = [&](int c) -> double {
+		c %= ncell;
+		if (c < 0) c += ncell;
+		return Un[(size_t)c * nel + e];
+	};
+	// sliding window U[c-2 .. c+2]
+	double um2 = U(c0 - 2), um1 = U(c0 - 1), u0 = U(c0), up1 = U(c0 + 1), up2 = U(c0 + 2);
+	for (int c = c0; c < c1; c++) {
+		const double am2 = um1 - um2, am1 = u0 - um1, a0 = up1 - u0, ap1 = up2 - up1;
+		const double bm1 = minmod_d(am1, am2), b0 = minmod_d(a0, am1), bp1 = minmod_d(ap1, a0);
+		const double r = u0 + scale * (
+			-A * (up1 - um1) + Aabs * (up1 - 2 * u0 + um1)
+			- S * (bp1 - bm1) + Sabs * (bp1 - 2 * b0 + bm1));
+		Un1[(size_t)c * nel + e] = r;
+		um2 = um1; um1 = u0; u0 = up1; up1 = up2;
+		up2 = U(c + 3);
+	}
+}
+
+// y = a * x + b * y  (elementwise, double)
+__global__ void k_axpby(const double* __restrict__ x, double* __restrict__ y, double a, double b, long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+} // namespace qbf

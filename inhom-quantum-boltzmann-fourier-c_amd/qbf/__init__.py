@@ -1,0 +1,216 @@
+"""Python host bindings of the MI355X collision-plus-transport engine (libqbf.so).
+
+A thin ctypes layer over the C API of include/qbf.h.  Device memory is owned by
+torch tensors on ``cuda:0``; the engine launches on torch's current stream.
+There is no CPU fallback: if ``lib/libqbf.so`` is missing or no HIP device is
+present, every entry point raises.
+
+Tensor layouts (see include/qbf.h):
+  Fourier state  complex128 [ncell, 4, N, N]     (wignerF_t per cell)
+  physical state float64    [ncell, 4, N, N]     (wignerV_t per cell)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libqbf.so")
+
+FP64, FP32 = 0, 1
+CC, CD, CFULL = 1, 2, 3
+TAB_I1, TAB_I4 = 0, 1
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+class QBFError(RuntimeError):
+    pass
+
+
+def build(verbose: bool = False) -> str:
+    """Compile csrc/ with hipcc for gfx950 into lib/libqbf.so (works without a GPU)."""
+    out = subprocess.run(["make", "-s", "-C", PKG_DIR], capture_output=not verbose, text=True)
+    if out.returncode != 0:
+        raise QBFError("build failed:\n" + (out.stdout or "") + (out.stderr or ""))
+    return LIB_PATH
+
+
+class _Tables(C.Structure):
+    _fields_ = [("numI1", C.c_uint), ("I1_psiR1", _dp), ("I1_psiR2", _dp),
+                ("numI2", C.c_uint), ("I2_psiR1", _dp), ("I2_psiR2", _dp),
+                ("numI3", C.c_uint), ("I3_psiR1", _dp), ("I3_psiR2", _dp),
+                ("numI4", C.c_uint), ("I4_psiR1", _dp), ("I4_psiR2", _dp)]
+
+
+def lib():
+    """Load libqbf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QBFError(f"{LIB_PATH} not found: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        vp, i, d, u = C.c_void_p, C.c_int, C.c_double, C.c_uint
+        sig = {
+            "qbf_last_error": (C.c_char_p, []),
+            "qbf_engine_create": (i, [i, i, i, C.POINTER(vp)]),
+            "qbf_engine_destroy": (None, [vp]),
+            "qbf_set_stream": (i, [vp, vp]),
+            "qbf_sync": (i, [vp]),
+            "qbf_set_quadrature": (i, [vp, C.POINTER(_Tables)]),
+            "qbf_set_i1_table": (i, [vp, i, u, _dp, _dp]),
+            "qbf_set_i2_table": (i, [vp, u, _dp, _dp]),
+            "qbf_set_i3_table": (i, [vp, u, _dp, _dp]),
+            "qbf_collision": (i, [vp, vp, i, i, vp, vp, vp, vp, d]),
+            "qbf_collision_step": (i, [vp, vp, i, d, vp, vp]),
+            "qbf_I1": (i, [vp, i, vp, vp, vp, vp]),
+            "qbf_I2": (i, [vp, vp, vp, vp, vp]),
+            "qbf_I3": (i, [vp, vp, vp, vp, vp]),
+            "qbf_phys_to_fourier": (i, [vp, vp, i, vp]),
+            "qbf_fourier_to_phys": (i, [vp, vp, i, vp]),
+            "qbf_transport_periodic": (i, [vp, vp, i, d, d, d, vp]),
+            "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise QBFError(lib().qbf_last_error().decode())
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def _np(a, dtype=np.float64):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(_dp)
+
+
+class Engine:
+    """One engine per (N, precision); holds quadrature tables and work buffers on the GPU."""
+
+    def __init__(self, N: int, precision: int = FP64, max_cells: int = 1):
+        import torch
+        if not torch.cuda.is_available():
+            raise QBFError("no HIP device available")
+        self.N, self.precision, self.max_cells = N, precision, max_cells
+        h = C.c_void_p()
+        _check(lib().qbf_engine_create(N, precision, max_cells, C.byref(h)))
+        self._h = h
+        self._torch = torch
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qbf_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --------------------------------------------------------------- stream
+    def _bind(self):
+        s = self._torch.cuda.current_stream().cuda_stream
+        _check(lib().qbf_set_stream(self._h, C.c_void_p(s)))
+
+    # --------------------------------------------------------------- tables (numpy, reference layout)
+    def set_quadrature(self, I1, I2, I3, I4):
+        keep = []
+        vals = []
+        for (p1, p2), cplx in ((I1, False), (I2, False), (I3, True), (I4, False)):
+            a1, q1 = _np(np.asarray(p1).view(np.float64) if cplx else p1)
+            a2, q2 = _np(p2)
+            keep += [a1, a2]
+            vals += [np.asarray(p2).shape[0], q1, q2]
+        t = _Tables(*vals)
+        _check(lib().qbf_set_quadrature(self._h, C.byref(t)))
+
+    def set_i1_table(self, slot, p1, p2):
+        a1, q1 = _np(p1)
+        a2, q2 = _np(p2)
+        _check(lib().qbf_set_i1_table(self._h, slot, a2.shape[0], q1, q2))
+
+    def set_i2_table(self, p1, p2):
+        a1, q1 = _np(p1)
+        a2, q2 = _np(p2)
+        _check(lib().qbf_set_i2_table(self._h, a1.shape[0], q1, q2))
+
+    def set_i3_table(self, p1, p2):
+        a1, q1 = _np(np.asarray(p1, np.complex128).view(np.float64))
+        a2, q2 = _np(p2)
+        _check(lib().qbf_set_i3_table(self._h, a2.shape[0], q1, q2))
+
+    # --------------------------------------------------------------- operators (torch cuda tensors)
+    def collision(self, W, mode=CFULL, base=None, Wcb=None, Bext=None, cf=1.0, out=None):
+        torch = self._torch
+        self._bind()
+        out = torch.empty_like(W) if out is None else out
+        _check(lib().qbf_collision(self._h, _ptr(W), W.shape[0], mode, _ptr(out), _ptr(base), _ptr(Wcb),
+                                   _ptr(Bext), cf))
+        return out
+
+    def collision_step(self, W, dt, Bext=None, out=None):
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_collision_step(self._h, _ptr(W), W.shape[0], dt, _ptr(Bext), _ptr(out)))
+        return out
+
+    def I1(self, f, g, h, slot=TAB_I1):
+        self._bind()
+        out = self._torch.empty_like(f)
+        _check(lib().qbf_I1(self._h, slot, _ptr(f), _ptr(g), _ptr(h), _ptr(out)))
+        return out
+
+    def I2(self, f, g, h):
+        self._bind()
+        out = self._torch.empty_like(f)
+        _check(lib().qbf_I2(self._h, _ptr(f), _ptr(g), _ptr(h), _ptr(out)))
+        return out
+
+    def I3(self, f, g, h):
+        self._bind()
+        out = self._torch.empty_like(f)
+        _check(lib().qbf_I3(self._h, _ptr(f), _ptr(g), _ptr(h), _ptr(out)))
+        return out
+
+    def phys_to_fourier(self, Wp):
+        self._bind()
+        out = self._torch.empty(Wp.shape, dtype=self._torch.complex128, device=Wp.device)
+        ng = Wp.numel() // (self.N * self.N)
+        _check(lib().qbf_phys_to_fourier(self._h, _ptr(Wp), ng, _ptr(out)))
+        return out
+
+    def fourier_to_phys(self, WF):
+        self._bind()
+        out = self._torch.empty(WF.shape, dtype=self._torch.float64, device=WF.device)
+        ng = WF.numel() // (self.N * self.N)
+        _check(lib().qbf_fourier_to_phys(self._h, _ptr(WF), ng, _ptr(out)))
+        return out
+
+    def transport_periodic(self, W, L, h, dt, out=None):
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_transport_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(out)))
+        return out
+
+    def inhom_step_periodic(self, W, L, h, dt, Bext=None, out=None):
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_inhom_step_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
+        return out
