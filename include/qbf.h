@@ -79,9 +79,26 @@ int qbf_fourier_to_phys(qbf_engine *e, const double *dFourier, int ngrid, double
 /* slope-limited transport, periodic in x, over ncell physical states (src/finite_volume.c:52-104) */
 int qbf_transport_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt, double *dWout);
 
+/* slope-limited transport of one periodic line of n cells with velocity A */
+int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A, double h, double dt, double *dOut);
+
 /* one Strang-split periodic step: half transport, collision, half transport (src/simulation.c:307-555) */
 int qbf_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
 	const double *dBext, double *dWnext);
+
+/* Timing of the dominant kernel (the I3 K-operator kernel): qbf_profile(e, 1) clears and
+ * starts recording HIP events around each of its launches on the engine stream;
+ * qbf_profile_read returns the summed kernel time, the launch count and the number of
+ * K_q applications (IFFT2 -> multiply -> FFT2 on a 2N x 2N grid) they performed. */
+int qbf_profile(qbf_engine *e, int enable);
+int qbf_profile_read(qbf_engine *e, double *ms_total, int *launches, double *kapps);
+
+/* quadrature tables in the reference's host layout, for runtime N (src/quadrature.c) */
+void qbf_legendre_gauss(unsigned M, double a, double b, double *x, double *w);
+void qbf_fourier_I1(int N, unsigned J, double L, double R, double *psiR1, double *psiR2);
+void qbf_fourier_I2(int N, unsigned J, double L, double R, double *psiR1, double *psiR2);
+void qbf_fourier_I3(int N, unsigned J, double L, double R, unsigned M, double *psiR1, double *psiR2);
+void qbf_fourier_I4(int N, unsigned J, double L, double R, double *psiR1, double *psiR2);
 
 /* device memory helpers for host code that does not include HIP headers */
 void *qbf_malloc(size_t bytes);

@@ -146,10 +146,33 @@ __device__ __forceinline__ void dft(cx<R>* v)
 	}
 }
 
+// Exchange a 32-bit value with lane (lane ^ m).  m is a compile-time constant after
+// unrolling: lane distances 1 and 2 use DPP quad permutes (VALU, no LDS traffic),
+// 4, 8, 16 the ds_swizzle xor mode, 32 a bpermute.
+__device__ __forceinline__ int xor_lane_i32(int x, int m)
+{
+	switch (m) {
+	case 1: return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+	case 2: return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+	case 4: return __builtin_amdgcn_ds_swizzle(x, 0x101F);               // and 0x1f, xor 4
+	case 8: return __builtin_amdgcn_ds_swizzle(x, 0x201F);
+	case 16: return __builtin_amdgcn_ds_swizzle(x, 0x401F);
+	default: return __shfl_xor(x, m);
+	}
+}
+__device__ __forceinline__ float xor_lane(float x, int m) { return __int_as_float(xor_lane_i32(__float_as_int(x), m)); }
+__device__ __forceinline__ double xor_lane(double x, int m)
+{
+	const long long b = __double_as_longlong(x);
+	const int lo = xor_lane_i32((int)(b & 0xffffffffLL), m);
+	const int hi = xor_lane_i32((int)(b >> 32), m);
+	return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 template <typename R>
 __device__ __forceinline__ cx<R> shfl_xor_c(cx<R> v, int m)
 {
-	return mk<R>(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
+	return mk<R>(xor_lane(v.x, m), xor_lane(v.y, m));
 }
 
 // Geometry of one P x P grid per workgroup of T threads.

@@ -48,20 +48,46 @@ static int set_err(int code, const char* fmt, ...)
 extern "C" const char* qbf_last_error(void) { return g_err.c_str(); }
 
 // threads per workgroup for a padded P x P grid (P = 2N), and for an unpadded N x N grid
+// XG: exchange rows and columns through a per-workgroup global scratch slot instead of LDS
+// (an FP64 128 x 128 grid is 256 KiB, more than the 160 KiB of LDS of a CU)
 template <typename R, int N> struct Cfg;
-template <> struct Cfg<double, 16> { static constexpr int T = 128; };
-template <> struct Cfg<double, 32> { static constexpr int T = 256; };
-template <> struct Cfg<float, 16> { static constexpr int T = 128; };
-template <> struct Cfg<float, 32> { static constexpr int T = 256; };
-template <> struct Cfg<float, 64> { static constexpr int T = 512; };
+template <> struct Cfg<double, 16> { static constexpr int T = 128; static constexpr bool XG = false; };
+template <> struct Cfg<double, 32> { static constexpr int T = 256; static constexpr bool XG = false; };
+template <> struct Cfg<double, 64> { static constexpr int T = 1024; static constexpr bool XG = true; };
+template <> struct Cfg<float, 16> { static constexpr int T = 128; static constexpr bool XG = false; };
+template <> struct Cfg<float, 32> { static constexpr int T = 256; static constexpr bool XG = false; };
+template <> struct Cfg<float, 64> { static constexpr int T = 512; static constexpr bool XG = false; };
 template <int N> struct CfgNN;
 template <> struct CfgNN<16> { static constexpr int T = 64; };
 template <> struct CfgNN<32> { static constexpr int T = 128; };
 template <> struct CfgNN<64> { static constexpr int T = 256; };
 
 struct qbf_engine {
-	virtual ~qbf_engine() {}
+	virtual ~qbf_engine()
+	{
+		for (auto& ev : evs) {
+			hipEventDestroy(ev.first);
+			hipEventDestroy(ev.second);
+		}
+	}
 	int gridN = 0, precision = 0, maxCells = 0;
+	// optional timing of the I3 kernel (k_i3acc) launches with HIP events on the engine stream
+	bool prof = false;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+	size_t nev = 0;
+	double profK = 0;                 // K_q applications issued while profiling
+	int prof_begin(hipEvent_t* b, hipEvent_t* e)
+	{
+		if (nev == evs.size()) {
+			hipEvent_t x, y;
+			if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) return -1;
+			evs.push_back({x, y});
+		}
+		*b = evs[nev].first;
+		*e = evs[nev].second;
+		nev++;
+		return 0;
+	}
 	hipStream_t stream = nullptr;
 	bool ownStream = false;
 	virtual int set_i1(int slot, unsigned num, const double* p1, const double* p2) = 0;
@@ -118,6 +144,7 @@ struct EngineT final : public qbf_engine {
 	static constexpr int P = 2 * N, NN = N * N, PP = P * P;
 	static constexpr int T = Cfg<R, N>::T;
 	static constexpr int TN = CfgNN<N>::T;
+	static constexpr bool XG = Cfg<R, N>::XG;
 	using GE = Geo<P, T>;
 	using C = cx<R>;
 	static constexpr size_t GB = sizeof(C) * PP;    // bytes of one padded grid
@@ -127,7 +154,7 @@ struct EngineT final : public qbf_engine {
 	int n2 = 0; DevBuf i2p1, i2p2;
 	int n3 = 0; DevBuf i3p1, i3p2n;
 
-	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, terms;
+	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, terms, xscr;
 	DevBuf Wmid, WF, WF2, Wt1, Wt2, src3;
 	size_t fBudget = (size_t)2 << 30;
 
@@ -225,6 +252,17 @@ struct EngineT final : public qbf_engine {
 	}
 
 	// --------------------------------------------------------------- launches
+	// exchange scratch for nblocks workgroups (only for configurations that exchange through global memory)
+	int xs_for(int nblocks, void** xs)
+	{
+		*xs = nullptr;
+		if (!XG) return 0;
+		int r = xscr.ensure((size_t)nblocks * P * GE::PITCH * sizeof(C));
+		if (r) return r;
+		*xs = xscr.p;
+		return 0;
+	}
+
 	int launch_padfft(const double2* src, int nsrc, const void* pre, int preKind, int q0, int nq, int nslot,
 		const int* srcOfSlot, void* out, int ncell)
 	{
@@ -233,7 +271,9 @@ struct EngineT final : public qbf_engine {
 		for (int i = 0; i < nslot; i++) a.srcOfSlot[i] = srcOfSlot[i];
 		a.out = out; a.tw256 = g_tw256;
 		const int nb = ncell * nq * nslot;
-		hipLaunchKernelGGL((k_padfft<R, N, T>), dim3(nb), dim3(T), 0, stream, a);
+		int r = xs_for(nb, &a.xs);
+		if (r) return r;
+		hipLaunchKernelGGL((k_padfft<R, N, T, XG>), dim3(nb), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		return 0;
 	}
@@ -248,7 +288,9 @@ struct EngineT final : public qbf_engine {
 		a.ninst = ninst;
 		for (int i = 0; i < ninst; i++) { a.instSrc[i] = instSrc[i]; a.instFh[i] = instFh[i]; }
 		a.out = out; a.tw256 = g_tw256;
-		hipLaunchKernelGGL((k_i1acc<R, N, T>), dim3(ncell * ninst), dim3(T), 0, stream, a);
+		int r = xs_for(ncell * ninst, &a.xs);
+		if (r) return r;
+		hipLaunchKernelGGL((k_i1acc<R, N, T, XG>), dim3(ncell * ninst), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		return 0;
 	}
@@ -261,7 +303,9 @@ struct EngineT final : public qbf_engine {
 		a.nterm = n2; a.ninst = ninst;
 		for (int i = 0; i < ninst; i++) { a.instF[i] = instF[i]; a.instG[i] = instG[i]; }
 		a.out = out; a.tw256 = g_tw256;
-		hipLaunchKernelGGL((k_i2acc<R, N, T>), dim3(ncell * ninst), dim3(T), 0, stream, a);
+		int r = xs_for(ncell * ninst, &a.xs);
+		if (r) return r;
+		hipLaunchKernelGGL((k_i2acc<R, N, T, XG>), dim3(ncell * ninst), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		return 0;
 	}
@@ -276,12 +320,12 @@ struct EngineT final : public qbf_engine {
 		int nchunk = (1024 + ncell * nout - 1) / (ncell * nout);
 		if (nchunk > nq) nchunk = nq;
 		if (nchunk < 1) nchunk = 1;
-		const size_t perQ = (size_t)ncell * nF * GB;
+		const size_t perQ = (size_t)ncell * nF * GB * (XG ? 2 : 1);
 		int qr = (int)(fBudget / perQ);
 		if (qr < 1) qr = 1;
 		if (qr > nq) qr = nq;
 		if (nchunk > qr) nchunk = qr;
-		int r = Fscr.ensure(perQ * qr);
+		int r = Fscr.ensure((size_t)ncell * nF * GB * qr);
 		if (!r) r = acc3.ensure((size_t)ncell * nout * nchunk * GB);
 		if (r) return r;
 		for (int q0 = 0; q0 < nq; q0 += qr) {
@@ -298,8 +342,18 @@ struct EngineT final : public qbf_engine {
 				}
 			}
 			a.nchunk = nchunk; a.accumulate = q0 > 0; a.acc = acc3.p; a.tw256 = g_tw256;
-			hipLaunchKernelGGL((k_i3acc<R, N, T>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			r = xs_for(ncell * nout * nchunk, &a.xs);
+			if (r) return r;
+			hipEvent_t eb = nullptr, ee = nullptr;
+			if (prof && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
+			hipLaunchKernelGGL((k_i3acc<R, N, T, XG>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			HIPCHK(hipGetLastError());
+			if (ee) {
+				HIPCHK(hipEventRecord(ee, stream));
+				double nk = 0;
+				for (int o = 0; o < nout; o++) nk += ninst[o];
+				profK += nk * ncell * nqr;
+			}
 		}
 		*nchunkOut = nchunk;
 		return 0;
@@ -321,7 +375,9 @@ struct EngineT final : public qbf_engine {
 		for (int o = 0; o < nout; o++) { a.nterm[o] = (int)tl[o].size(); a.outComp[o] = outComp[o]; }
 		a.outScale = 1.0; a.out = out; a.base = base; a.Wcb = Bext ? Wcb : nullptr; a.Bext = Bext; a.cf = cf;
 		a.tw256 = g_tw256;
-		hipLaunchKernelGGL((k_combine<R, N, T>), dim3(ncell * nout), dim3(T), 0, stream, a);
+		r = xs_for(ncell * nout, &a.xs);
+		if (r) return r;
+		hipLaunchKernelGGL((k_combine<R, N, T, XG>), dim3(ncell * nout), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		// the host term table must outlive the asynchronous copy
 		HIPCHK(hipStreamSynchronize(stream));
@@ -565,6 +621,7 @@ int qbf_engine_create(int N, int precision, int maxCells, qbf_engine** out)
 	int r;
 	if (precision == QBF_FP64 && N == 16) r = make_engine<double, 16>(maxCells, &e);
 	else if (precision == QBF_FP64 && N == 32) r = make_engine<double, 32>(maxCells, &e);
+	else if (precision == QBF_FP64 && N == 64) r = make_engine<double, 64>(maxCells, &e);
 	else if (precision == QBF_FP32 && N == 16) r = make_engine<float, 16>(maxCells, &e);
 	else if (precision == QBF_FP32 && N == 32) r = make_engine<float, 32>(maxCells, &e);
 	else if (precision == QBF_FP32 && N == 64) r = make_engine<float, 64>(maxCells, &e);
@@ -606,6 +663,31 @@ int qbf_set_stream(qbf_engine* e, void* s)
 		HIPCHK(hipStreamCreate(&e->stream));
 		e->ownStream = true;
 	}
+	return 0;
+}
+
+int qbf_profile(qbf_engine* e, int enable)
+{
+	if (!e) return set_err(-1, "null engine");
+	e->prof = enable != 0;
+	e->nev = 0;
+	e->profK = 0;
+	return 0;
+}
+
+int qbf_profile_read(qbf_engine* e, double* ms_total, int* launches, double* kapps)
+{
+	if (!e) return set_err(-1, "null engine");
+	HIPCHK(hipStreamSynchronize(e->stream));
+	double tot = 0;
+	for (size_t i = 0; i < e->nev; i++) {
+		float ms = 0;
+		HIPCHK(hipEventElapsedTime(&ms, e->evs[i].first, e->evs[i].second));
+		tot += ms;
+	}
+	*ms_total = tot;
+	*launches = (int)e->nev;
+	*kapps = e->profK;
 	return 0;
 }
 
@@ -696,6 +778,14 @@ int qbf_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L,
 {
 	if (!e || !W || !out) return set_err(-1, "null argument");
 	return e->inhom_step(W, ncell, L, h, dt, Bext, out);
+}
+
+int qbf_transport_line_periodic(qbf_engine* e, const double* U, int n, double A, double h, double dt, double* out)
+{
+	if (!e || !U || !out || n < 1) return set_err(-1, "bad argument");
+	hipLaunchKernelGGL(k_transport_line, dim3((n + 255) / 256), dim3(256), 0, e->stream, U, out, n, A, h, dt);
+	HIPCHK(hipGetLastError());
+	return 0;
 }
 
 void* qbf_malloc(size_t bytes)

@@ -87,14 +87,15 @@ __device__ __forceinline__ void fft2_reg(cx<R>* v, const Xpose<R, GE>& X, const 
 }
 
 // ---------------------------------------------------------------------------------------------
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_padfft(PadFFTArgs a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
 	__shared__ cx<R> tw[P];
 	load_twiddles<R, P, T>(tw, a.tw256);
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
 	const int o = blockIdx.x;
 	const int slot = o % a.nslot;
 	const int qq = (o / a.nslot) % a.nq;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(T) void k_padfft(PadFFTArgs a)
 	if (a.preKind == 1) pre = (const double*)a.pre + (size_t)(a.q0 + qq) * (N * N);
 	if (a.preKind == 2) pre = (const double2*)a.pre + (size_t)(a.q0 + qq) * (N * N);
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
-	Xpose<R, GE> X{lds};
+	Xpose<R, GE> X{xbuf};
 	cx<R> v[E];
 	__syncthreads();
 	padfft_reg<R, N, GE>(v, src, pre, a.preKind, X, tw, line, p);
@@ -112,21 +113,22 @@ __global__ __launch_bounds__(T) void k_padfft(PadFFTArgs a)
 }
 
 // ---------------------------------------------------------------------------------------------
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
 	__shared__ cx<R> tw[P];
 	load_twiddles<R, P, T>(tw, a.tw256);
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
 	const int inst = blockIdx.x % a.ninst;
 	const int cell = blockIdx.x / a.ninst;
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instSrc[inst]) * (N * N);
 	const cx<R>* Fh = (const cx<R>*)a.Fhat + ((size_t)cell * a.nFh + a.instFh[inst]) * (P * P);
 	const R fa = (R)a.fa, fb = (R)a.fb;
-	Xpose<R, GE> X{lds};
+	Xpose<R, GE> X{xbuf};
 	cx<R> acc[E], v[E];
 #pragma unroll
 	for (int s = 0; s < E; s++) acc[s] = czero<R>();
@@ -153,20 +155,21 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 }
 
 // ---------------------------------------------------------------------------------------------
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
 	__shared__ cx<R> tw[P];
 	load_twiddles<R, P, T>(tw, a.tw256);
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
 	const int inst = blockIdx.x % a.ninst;
 	const int cell = blockIdx.x / a.ninst;
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const double2* f = a.src + ((size_t)cell * a.nsrc + a.instF[inst]) * (N * N);
 	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instG[inst]) * (N * N);
-	Xpose<R, GE> X{lds};
+	Xpose<R, GE> X{xbuf};
 	cx<R> acc[E], x1[E], v[E];
 #pragma unroll
 	for (int u = 0; u < E; u++) acc[u] = czero<R>();
@@ -185,14 +188,15 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 
 // ---------------------------------------------------------------------------------------------
 // The dominant kernel: 16 applications of K_q per quadrature term and cell for CdInt.
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
 	__shared__ cx<R> tw[P];
 	load_twiddles<R, P, T>(tw, a.tw256);
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
 	const int chunk = blockIdx.x % a.nchunk;
 	const int out = (blockIdx.x / a.nchunk) % a.nout;
 	const int cell = blockIdx.x / (a.nchunk * a.nout);
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 	const int qa = (int)(((long long)chunk * a.nq) / a.nchunk);
 	const int qb = (int)(((long long)(chunk + 1) * a.nq) / a.nchunk);
 	cx<R>* accp = (cx<R>*)a.acc + (((size_t)cell * a.nout + out) * a.nchunk + chunk) * (P * P);
-	Xpose<R, GE> X{lds};
+	Xpose<R, GE> X{xbuf};
 	cx<R> acc[E], v[E];
 	if (a.accumulate) {
 		load_native<R, GE>(acc, accp);
@@ -242,19 +246,20 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 }
 
 // ---------------------------------------------------------------------------------------------
-template <typename R, int N, int T>
+template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E, G = GE::G;
-	__shared__ cx<R> lds[P * GE::PITCH];
+	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
 	__shared__ cx<R> tw[P];
 	load_twiddles<R, P, T>(tw, a.tw256);
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
 	const int o = blockIdx.x % a.nout;
 	const int cell = blockIdx.x / a.nout;
 	const int line = threadIdx.x / G, p = threadIdx.x % G;
 	const int t = threadIdx.x;
-	Xpose<R, GE> X{lds};
+	Xpose<R, GE> X{xbuf};
 	cx<R> v[E];
 #pragma unroll
 	for (int u = 0; u < E; u++) v[u] = czero<R>();
@@ -416,6 +421,28 @@ __global__ __launch_bounds__(256) void k_transport_periodic(const double* __rest
 		um2 = um1; um1 = u0; u0 = up1; up1 = up2;
 		up2 = U(c + 3);
 	}
+}
+
+// One line of n cells with velocity A (the single-line entry point of the reference,
+// src/finite_volume.c:52-104); same arithmetic as k_transport_periodic.
+__global__ __launch_bounds__(256) void k_transport_line(const double* __restrict__ Un, double* __restrict__ Un1,
+	int n, double A, double h, double dt)
+{
+#pragma clang fp contract(off)
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= n) return;
+	const double Aabs = fabs(A);
+	const double nu = dt * A / h;
+	const double S = 0.5 * A * (copysign(1.0, nu) - nu);
+	const double Sabs = 0.5 * Aabs * (copysign(1.0, nu) - nu);
+	const double scale = dt / (2 * h);
+	auto U = [&](int i) -> double { i %= n; if (i < 0) i += n; return Un[i]; };
+	const double um2 = U(c - 2), um1 = U(c - 1), u0 = U(c), up1 = U(c + 1), up2 = U(c + 2);
+	const double am2 = um1 - um2, am1 = u0 - um1, a0 = up1 - u0, ap1 = up2 - up1;
+	const double bm1 = minmod_d(am1, am2), b0 = minmod_d(a0, am1), bp1 = minmod_d(ap1, a0);
+	Un1[c] = u0 + scale * (
+		-A * (up1 - um1) + Aabs * (up1 - 2 * u0 + um1)
+		- S * (bp1 - bm1) + Sabs * (bp1 - 2 * b0 + bm1));
 }
 
 // y = a * x + b * y  (elementwise, double)

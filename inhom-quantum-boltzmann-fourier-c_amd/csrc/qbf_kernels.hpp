@@ -10,6 +10,7 @@ namespace qbf {
 // o = (cell * nq + qq) * nslot + slot, src grid = src[cell * nsrc + srcOfSlot[slot]],
 // pre = pre[q0 + qq] (none / real N*N / complex N*N).
 struct PadFFTArgs {
+	void* xs;             // per-workgroup exchange scratch (global), or null for LDS
 	const double2* src;
 	int nsrc;
 	const void* pre;
@@ -24,6 +25,7 @@ struct PadFFTArgs {
 // I1-type accumulation (I1 for Cc, I1 with FourierI4 tables for I4):
 //   out[cell][inst] = FFT2( sum_g psi1_g .* IFFT2( (fa*Fhat + fb) .* FFT2(pad(phi_g .* src)) ) )
 struct I1Args {
+	void* xs;             // per-workgroup exchange scratch (global), or null for LDS
 	const double2* src;   // [ncell][nsrc][N*N]
 	int nsrc;
 	const void* Fhat;     // [ncell][nFh][P*P] R_b native
@@ -41,6 +43,7 @@ struct I1Args {
 
 // I2 accumulation: out[cell][inst] = sum_q FFT2(pad(psi1_q .* f)) .* FFT2(pad(psi2_q .* g))
 struct I2Args {
+	void* xs;             // per-workgroup exchange scratch (global), or null for LDS
 	const double2* src;
 	int nsrc;
 	const double* psi1;   // [nterm][N*N]
@@ -57,6 +60,7 @@ struct I2Args {
 //   acc[cell][out][chunk] (+)= sum_{qq in chunk} sum_inst w * F[f] .* K_q[ Shat[g] .* F[h] ],
 //   K_q[X] = FFT2( psi2_q .* IFFT2(X) ).
 struct I3Args {
+	void* xs;             // per-workgroup exchange scratch (global), or null for LDS
 	const void* Shat;     // [ncell][nS][P*P] R_b native
 	int nS;
 	const void* F;        // [ncell][nq][nF][P*P] R_b native
@@ -87,6 +91,7 @@ struct CTerm {
 // Combine terms per output component, IFFT2, extract the N x N block, then
 //   out = base + cf * (outScale * value + CB(Wcb, Bext)).
 struct CombineArgs {
+	void* xs;             // per-workgroup exchange scratch (global), or null for LDS
 	const CTerm* terms;   // [nout][maxTerms]
 	int maxTerms;
 	int nterm[4];

@@ -74,6 +74,13 @@ def lib():
             "qbf_fourier_to_phys": (i, [vp, vp, i, vp]),
             "qbf_transport_periodic": (i, [vp, vp, i, d, d, d, vp]),
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
+            "qbf_profile": (i, [vp, i]),
+            "qbf_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]),
+            "qbf_legendre_gauss": (None, [u, d, d, _dp, _dp]),
+            "qbf_fourier_I1": (None, [i, u, d, d, _dp, _dp]),
+            "qbf_fourier_I2": (None, [i, u, d, d, _dp, _dp]),
+            "qbf_fourier_I3": (None, [i, u, d, d, u, _dp, _dp]),
+            "qbf_fourier_I4": (None, [i, u, d, d, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -100,6 +107,30 @@ def _np(a, dtype=np.float64):
     return a, a.ctypes.data_as(_dp)
 
 
+def quadrature(N: int, J: int, L: float, R: float, M: int, J1: int | None = None):
+    """The four quadrature tables (FourierI1..I4 of the reference, src/quadrature.c) as numpy
+    arrays in the reference's layout, computed by the library's host code."""
+    J1 = J if J1 is None else J1
+    P = 2 * N
+    L_ = lib()
+    i1 = (np.empty((J1 * J1, P, P)), np.empty((J1 * J1, N, N)))
+    L_.qbf_fourier_I1(N, J1, L, R, i1[0].ctypes.data_as(_dp), i1[1].ctypes.data_as(_dp))
+    i2 = (np.empty((J, N, N)), np.empty((J, N, N)))
+    L_.qbf_fourier_I2(N, J, L, R, i2[0].ctypes.data_as(_dp), i2[1].ctypes.data_as(_dp))
+    i3 = (np.empty((J * M, N, N), np.complex128), np.empty((J * M, P, P)))
+    L_.qbf_fourier_I3(N, J, L, R, M, i3[0].view(np.float64).ctypes.data_as(_dp), i3[1].ctypes.data_as(_dp))
+    i4 = (np.empty((J, P, P)), np.empty((J, N, N)))
+    L_.qbf_fourier_I4(N, J, L, R, i4[0].ctypes.data_as(_dp), i4[1].ctypes.data_as(_dp))
+    return i1, i2, i3, i4
+
+
+def legendre_gauss(M: int, a: float, b: float):
+    x = np.empty(M)
+    w = np.empty(M)
+    lib().qbf_legendre_gauss(M, a, b, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    return x, w
+
+
 class Engine:
     """One engine per (N, precision); holds quadrature tables and work buffers on the GPU."""
 
@@ -123,6 +154,15 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+    # --------------------------------------------------------------- profiling of the I3 kernel
+    def profile(self, enable: bool = True):
+        _check(lib().qbf_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        ms, n, k = C.c_double(), C.c_int(), C.c_double()
+        _check(lib().qbf_profile_read(self._h, C.byref(ms), C.byref(n), C.byref(k)))
+        return ms.value, n.value, k.value
 
     # --------------------------------------------------------------- stream
     def _bind(self):

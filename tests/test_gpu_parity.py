@@ -153,7 +153,7 @@ def test_phys_fourier_roundtrip(eng32):
 
 
 @pytest.mark.parametrize("N,prec,tol", [(16, qbf.FP64, TOL64), (16, qbf.FP32, TOL32), (32, qbf.FP32, TOL32),
-                                        (64, qbf.FP32, TOL32)])
+                                        (64, qbf.FP32, TOL32), (64, qbf.FP64, TOL64)])
 def test_random_collision_step_vs_oracle(N, prec, tol):
     rng = np.random.default_rng(100 + N + prec)
     q = O.Quad(N, 4, 6.0, 3.75, 3)
