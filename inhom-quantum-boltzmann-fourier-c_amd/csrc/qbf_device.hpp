@@ -182,32 +182,72 @@ struct Geo {
 	static constexpr int T = T_;
 	static constexpr int G = T_ / P_;       // threads per line
 	static constexpr int E = P_ / G;        // elements per thread
-	static constexpr int PAD = 1;           // LDS row padding (elements)
-	static constexpr int PITCH = P_ + PAD;
+	static constexpr int PITCH = P_ + G;    // LDS row pitch (elements), see Xpose
 	static_assert(G * P_ == T_, "T must be a multiple of P");
 	static_assert(G * E == P_, "G must divide P");
 	static_assert(G <= 64 && (64 % G) == 0, "a line must sit inside one wavefront");
 };
 
+// An SGPR zero the compiler cannot see through: adding it to an LDS index keeps table loads
+// next to their use instead of hoisting them (and their registers) out of the term loops.
+__device__ __forceinline__ int opaque_zero()
+{
+	int z = 0;
+	asm volatile("" : "+s"(z));
+	return z;
+}
+
+// p + off with an opaque SGPR term, so the address (and the load) is formed next to its
+// use; p keeps its address space (a global pointer stays a global load).
+template <typename X>
+__device__ __forceinline__ X* uni(X* p, long long off)
+{
+	typedef __attribute__((address_space(1))) X gX;
+	unsigned long long a = (unsigned long long)(p + off);
+	asm volatile("" : "+s"(a));
+	return (X*)(gX*)a;
+}
+
+// Materialise a value in VGPRs here: keeps a load unconditional (not sunk into a branch
+// of the lanes that use it).
+template <typename R>
+__device__ __forceinline__ cx<R> pin(cx<R> w)
+{
+	asm volatile("" : "+v"(w.x), "+v"(w.y));
+	return w;
+}
+
 // Forward FFT of one line: interleaved layout in, blocked layout out.
-// twP[j] = (cos, sin)(2 pi j / P), j < P (in LDS).
+// twP[j] = (cos, sin)(2 pi j / P), j < P; twP[P + p*E + u] = (cos, sin)(2 pi p u / P) (in LDS).
 template <typename R, int P, int G, int E>
 __device__ __forceinline__ void line_fwd(cx<R>* v, int p, const cx<R>* twP)
 {
 	dft<R, E, -1>(v);
 	if constexpr (G > 1) {
+		const cx<R>* tl = twP + P + p * E + opaque_zero();
 #pragma unroll
-		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], twP[p * u]);
+		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], tl[u]);
 #pragma unroll
 		for (int h = G / 2; h >= 1; h >>= 1) {
+			// DIF butterfly across lanes p, p^h:  lo: a + b,  hi: (a - b) * conj(w)
+			// as  (partner + sgn * self) * wl  with lane constants sgn = +-1, wl = 1 or conj(w)
 			const bool hi = (p & h) != 0;
-			const cx<R> w = twP[(p & (h - 1)) * (P / (2 * h))];
+			const R sgn = hi ? R(-1) : R(1);
+			if (h == 1) {                      // twiddle exp(0) = 1
 #pragma unroll
-			for (int u = 0; u < E; u++) {
-				const cx<R> pv = shfl_xor_c<R>(v[u], h);
-				const cx<R> d = cmulc<R>(csub<R>(pv, v[u]), w);
-				const cx<R> s = cadd<R>(v[u], pv);
-				v[u] = hi ? d : s;
+				for (int u = 0; u < E; u++) {
+					const cx<R> pv = shfl_xor_c<R>(v[u], h);
+					v[u] = mk<R>(pv.x + sgn * v[u].x, pv.y + sgn * v[u].y);
+				}
+			} else {
+				const cx<R> w = pin<R>(twP[(p & (h - 1)) * (P / (2 * h)) + opaque_zero()]);
+				const cx<R> wl = hi ? mk<R>(w.x, -w.y) : mk<R>(R(1), R(0));
+#pragma unroll
+				for (int u = 0; u < E; u++) {
+					const cx<R> pv = shfl_xor_c<R>(v[u], h);
+					const cx<R> t = mk<R>(pv.x + sgn * v[u].x, pv.y + sgn * v[u].y);
+					v[u] = cmul<R>(t, wl);
+				}
 			}
 		}
 	}
@@ -220,37 +260,59 @@ __device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
 	if constexpr (G > 1) {
 #pragma unroll
 		for (int h = 1; h < G; h <<= 1) {
+			// DIT butterfly across lanes p, p^h:  lo: a + w b,  hi: a - w b  (a on lo, b on hi):
+			// the hi lane pre-multiplies its own value by w, then  partner' + sgn * self'
 			const bool hi = (p & h) != 0;
-			const cx<R> w = twP[(p & (h - 1)) * (P / (2 * h))];
+			const R sgn = hi ? R(-1) : R(1);
+			if (h == 1) {                      // twiddle exp(0) = 1
 #pragma unroll
-			for (int u = 0; u < E; u++) {
-				const cx<R> pv = shfl_xor_c<R>(v[u], h);
-				const cx<R> a = hi ? pv : v[u];
-				const cx<R> b = hi ? v[u] : pv;
-				const cx<R> wb = cmul<R>(b, w);
-				v[u] = hi ? csub<R>(a, wb) : cadd<R>(a, wb);
+				for (int u = 0; u < E; u++) {
+					const cx<R> pv = shfl_xor_c<R>(v[u], h);
+					v[u] = mk<R>(pv.x + sgn * v[u].x, pv.y + sgn * v[u].y);
+				}
+			} else {
+				const cx<R> w = pin<R>(twP[(p & (h - 1)) * (P / (2 * h)) + opaque_zero()]);
+				const cx<R> wl = hi ? w : mk<R>(R(1), R(0));
+#pragma unroll
+				for (int u = 0; u < E; u++) {
+					const cx<R> sv = cmul<R>(v[u], wl);
+					const cx<R> pv = shfl_xor_c<R>(sv, h);
+					v[u] = mk<R>(pv.x + sgn * sv.x, pv.y + sgn * sv.y);
+				}
 			}
 		}
+		const cx<R>* tl = twP + P + p * E + opaque_zero();
 #pragma unroll
-		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], twP[p * u]);
+		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], tl[u]);
 	}
 	dft<R, E, +1>(v);
 }
 
-// LDS exchanges.  The grid is stored row-major with a padded pitch.
+// LDS exchanges.  The grid is stored row-major with pitch P + G and a skew of SKEW elements
+// per block of E rows.  With 64 banks of 4 bytes one access phase serves LANES lanes
+// (256 bytes).  Writes / reads of rows (R_i) hit LANES/G consecutive rows times G
+// consecutive columns: distinct banks because the pitch is G modulo LANES.  Writes / reads
+// of columns (C_b) hit LANES/G consecutive columns in G row blocks E rows apart: E*pitch is
+// a multiple of LANES, and the per-block skew LANES/G separates the blocks.
 template <typename R, class GE>
 struct Xpose {
-	cx<R>* g;
-	__device__ __forceinline__ int a(int row, int col) const { return row * GE::PITCH + col; }
+	static constexpr int LANES = 256 / (int)sizeof(cx<R>);
+	static constexpr int SKEW = LANES / GE::G;
+	static constexpr int SIZE = GE::P * GE::PITCH + GE::G * SKEW;
+	cx<R>* g0;
+	__device__ __forceinline__ cx<R>* gp() const { return g0 + opaque_zero(); }
+	__device__ __forceinline__ int a(int row, int col) const { return row * GE::PITCH + col + (row / GE::E) * SKEW; }
 	// R_i (line = row) -> LDS
 	__device__ __forceinline__ void put_Ri(const cx<R>* v, int line, int p) const
 	{
+		cx<R>* g = gp();
 #pragma unroll
 		for (int s = 0; s < GE::E; s++) g[a(line, p + GE::G * s)] = v[s];
 	}
 	// LDS -> C_b (line = column)
 	__device__ __forceinline__ void get_Cb(cx<R>* v, int line, int p) const
 	{
+		const cx<R>* g = gp();
 		const int r0 = GE::E * brev<GE::G>(p);
 #pragma unroll
 		for (int u = 0; u < GE::E; u++) v[u] = g[a(r0 + u, line)];
@@ -258,6 +320,7 @@ struct Xpose {
 	// C_b (line = column) -> LDS
 	__device__ __forceinline__ void put_Cb(const cx<R>* v, int line, int p) const
 	{
+		cx<R>* g = gp();
 		const int r0 = GE::E * brev<GE::G>(p);
 #pragma unroll
 		for (int u = 0; u < GE::E; u++) g[a(r0 + u, line)] = v[u];
@@ -265,18 +328,21 @@ struct Xpose {
 	// LDS -> R_i (line = row)
 	__device__ __forceinline__ void get_Ri(cx<R>* v, int line, int p) const
 	{
+		const cx<R>* g = gp();
 #pragma unroll
 		for (int s = 0; s < GE::E; s++) v[s] = g[a(line, p + GE::G * s)];
 	}
 };
 
-// Fill the LDS twiddle table twP[j] = (cos, sin)(2 pi j / P) from a global table
-// of (cos, sin)(2 pi j / 256) in double.
+// Fill the LDS twiddle tables from a global table of (cos, sin)(2 pi j / 256) in double:
+// twP[j] = w^j for j < P, and the per-lane table twP[P + p*E + u] = w^(p*u), w = exp(2 pi i / P).
 template <typename R, int P, int T>
 __device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
 {
-	for (int j = threadIdx.x; j < P; j += T) {
-		const double2 w = tw256[j * (256 / P)];
+	constexpr int G = T / P, E = P / G;
+	for (int j = threadIdx.x; j < 2 * P; j += T) {
+		const int k = j < P ? j : ((j - P) / E) * ((j - P) % E);
+		const double2 w = tw256[k * (256 / P)];
 		twP[j] = mk<R>((R)w.x, (R)w.y);
 	}
 }
@@ -286,13 +352,13 @@ template <typename R, class GE>
 __device__ __forceinline__ void store_native(cx<R>* dst, const cx<R>* v)
 {
 #pragma unroll
-	for (int u = 0; u < GE::E; u++) dst[u * GE::T + threadIdx.x] = v[u];
+	for (int u = 0; u < GE::E; u++) uni(dst, (long long)u * GE::T)[threadIdx.x] = v[u];
 }
 template <typename R, class GE>
 __device__ __forceinline__ void load_native(cx<R>* v, const cx<R>* src)
 {
 #pragma unroll
-	for (int u = 0; u < GE::E; u++) v[u] = src[u * GE::T + threadIdx.x];
+	for (int u = 0; u < GE::E; u++) v[u] = uni(src, (long long)u * GE::T)[threadIdx.x];
 }
 
 } // namespace qbf

@@ -56,7 +56,7 @@ template <> struct Cfg<double, 32> { static constexpr int T = 256; static conste
 template <> struct Cfg<double, 64> { static constexpr int T = 1024; static constexpr bool XG = true; };
 template <> struct Cfg<float, 16> { static constexpr int T = 128; static constexpr bool XG = false; };
 template <> struct Cfg<float, 32> { static constexpr int T = 256; static constexpr bool XG = false; };
-template <> struct Cfg<float, 64> { static constexpr int T = 512; static constexpr bool XG = false; };
+template <> struct Cfg<float, 64> { static constexpr int T = 1024; static constexpr bool XG = false; };
 template <int N> struct CfgNN;
 template <> struct CfgNN<16> { static constexpr int T = 64; };
 template <> struct CfgNN<32> { static constexpr int T = 128; };
@@ -257,7 +257,7 @@ struct EngineT final : public qbf_engine {
 	{
 		*xs = nullptr;
 		if (!XG) return 0;
-		int r = xscr.ensure((size_t)nblocks * P * GE::PITCH * sizeof(C));
+		int r = xscr.ensure((size_t)nblocks * Xpose<R, GE>::SIZE * sizeof(C));
 		if (r) return r;
 		*xs = xscr.p;
 		return 0;

@@ -92,10 +92,10 @@ __global__ __launch_bounds__(T) void k_padfft(PadFFTArgs a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
-	__shared__ cx<R> tw[P];
+	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
+	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
-	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int o = blockIdx.x;
 	const int slot = o % a.nslot;
 	const int qq = (o / a.nslot) % a.nq;
@@ -118,10 +118,10 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
-	__shared__ cx<R> tw[P];
+	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
+	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
-	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int inst = blockIdx.x % a.ninst;
 	const int cell = blockIdx.x / a.ninst;
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 		padfft_reg<R, N, GE>(v, g, a.phi + (size_t)q * (N * N), 1, X, tw, line, p);
 #pragma unroll
 		for (int u = 0; u < E; u++) {
-			cx<R> f = Fh[u * T + threadIdx.x];
+			cx<R> f = uni(Fh, (long long)u * T)[threadIdx.x];
 			f = mk<R>(fa * f.x + fb, fa * f.y);
 			v[u] = cmul<R>(f, v[u]);
 		}
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 		const R* ps = (const R*)a.psi1n + (size_t)q * (P * P);
 #pragma unroll
 		for (int s = 0; s < E; s++) {
-			const R w = ps[s * T + threadIdx.x];
+			const R w = uni(ps, (long long)s * T)[threadIdx.x];
 			acc[s].x += w * v[s].x;
 			acc[s].y += w * v[s].y;
 		}
@@ -160,10 +160,10 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
-	__shared__ cx<R> tw[P];
+	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
+	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
-	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int inst = blockIdx.x % a.ninst;
 	const int cell = blockIdx.x / a.ninst;
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
@@ -193,10 +193,10 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
-	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
-	__shared__ cx<R> tw[P];
+	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
+	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
-	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int chunk = blockIdx.x % a.nchunk;
 	const int out = (blockIdx.x / a.nchunk) % a.nout;
 	const int cell = blockIdx.x / (a.nchunk * a.nout);
@@ -225,21 +225,26 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 			const cx<R>* Ff = Fq + (size_t)a.instF[out][in] * (P * P);
 			const R w = (R)a.instW[out][in];
 #pragma unroll
-			for (int u = 0; u < E; u++) v[u] = cmul<R>(Sg[u * T + t], Fh[u * T + t]);
+			for (int u = 0; u < E; u++) v[u] = cmul<R>(uni(Sg, (long long)u * T)[t], uni(Fh, (long long)u * T)[t]);
 			ifft2_reg<R, GE>(v, X, tw, line, p);
+			// keep the table and operand loads next to their use: hoisting them above the
+			// line FFTs would keep E extra complex values live and spill the accumulator
+			__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 			for (int s = 0; s < E; s++) {
-				const R m = ps[s * T + t];
+				const R m = uni(ps, (long long)s * T)[t];
 				v[s].x *= m;
 				v[s].y *= m;
 			}
 			fft2_reg<R, GE>(v, X, tw, line, p);
+			__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 			for (int u = 0; u < E; u++) {
-				const cx<R> y = cmul<R>(Ff[u * T + t], v[u]);
+				const cx<R> y = cmul<R>(uni(Ff, (long long)u * T)[t], v[u]);
 				acc[u].x += w * y.x;
 				acc[u].y += w * y.y;
 			}
+			__builtin_amdgcn_sched_barrier(0);
 		}
 	}
 	store_native<R, GE>(accp, acc);
@@ -251,10 +256,10 @@ __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 {
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E, G = GE::G;
-	__shared__ cx<R> lds[XG ? 1 : P * GE::PITCH];
-	__shared__ cx<R> tw[P];
+	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
+	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
-	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * (P * GE::PITCH) : lds;
+	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int o = blockIdx.x % a.nout;
 	const int cell = blockIdx.x / a.nout;
 	const int line = threadIdx.x / G, p = threadIdx.x % G;
@@ -328,8 +333,8 @@ __global__ __launch_bounds__(T) void k_phys2fourier(const double* __restrict__ i
 {
 	using GE = Geo<N, T>;
 	constexpr int E = GE::E, G = GE::G;
-	__shared__ cx<double> lds[N * GE::PITCH];
-	__shared__ cx<double> tw[N];
+	__shared__ cx<double> lds[Xpose<double, GE>::SIZE];
+	__shared__ cx<double> tw[2 * N];
 	load_twiddles<double, N, T>(tw, tw256);
 	const int gi = blockIdx.x;
 	if (gi >= ngrid) return;
@@ -354,8 +359,8 @@ __global__ __launch_bounds__(T) void k_fourier2phys(const double2* __restrict__ 
 {
 	using GE = Geo<N, T>;
 	constexpr int E = GE::E, G = GE::G;
-	__shared__ cx<double> lds[N * GE::PITCH];
-	__shared__ cx<double> tw[N];
+	__shared__ cx<double> lds[Xpose<double, GE>::SIZE];
+	__shared__ cx<double> tw[2 * N];
 	load_twiddles<double, N, T>(tw, tw256);
 	const int gi = blockIdx.x;
 	if (gi >= ngrid) return;
