@@ -86,6 +86,16 @@ int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A
 int qbf_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
 	const double *dBext, double *dWnext);
 
+/* FP64 conservation correction of a collision step (physical states, [ncell][4][N][N]):
+ * Wnew is replaced by Wold + P(Wnew - Wold), where P removes, per cell, the selected moments
+ * of the increment with a |w0|-weighted least-squares correction.  mask: QBF_CONS_MASS (mass
+ * and the three spin densities), QBF_CONS_MOMENTUM, QBF_CONS_ENERGY.
+ * qbf_set_moment_correction(e, mask) applies it inside qbf_inhom_step_periodic after the
+ * collision (0 = off, the reference's behaviour). */
+enum { QBF_CONS_MASS = 1, QBF_CONS_MOMENTUM = 2, QBF_CONS_ENERGY = 4 };
+int qbf_set_moment_correction(qbf_engine *e, int mask);
+int qbf_moment_correct(qbf_engine *e, const double *dWold, double *dWnew, int ncell, double L, int mask);
+
 /* Timing of the dominant kernel (the I3 K-operator kernel): qbf_profile(e, 1) clears and
  * starts recording HIP events around each of its launches on the engine stream;
  * qbf_profile_read returns the summed kernel time, the launch count and the number of

@@ -71,6 +71,7 @@ struct qbf_engine {
 		}
 	}
 	int gridN = 0, precision = 0, maxCells = 0;
+	int momentMask = 0;               // conservation correction after each collision step (0 = off)
 	// optional timing of the I3 kernel (k_i3acc) launches with HIP events on the engine stream
 	bool prof = false;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
@@ -103,6 +104,12 @@ struct qbf_engine {
 	virtual int fourier2phys(const double2* in, int ngrid, double* out) = 0;
 	virtual int transport(const double* W, int ncell, double L, double h, double dt, double* out) = 0;
 	virtual int inhom_step(const double* W, int ncell, double L, double h, double dt, const double* Bext, double* out) = 0;
+	int moment_correct(const double* Wold, double* Wnew, int ncell, double L, int mask)
+	{
+		hipLaunchKernelGGL(k_moment_correct, dim3(ncell), dim3(256), 0, stream, Wold, Wnew, gridN, L, mask);
+		hipError_t e = hipGetLastError();
+		return e == hipSuccess ? 0 : -2;
+	}
 };
 
 namespace {
@@ -587,6 +594,7 @@ struct EngineT final : public qbf_engine {
 		if (!r) r = phys2fourier(Wt1.template as<double>(), 4 * ncell, WF.template as<double2>());
 		if (!r) r = collision_step(WF.template as<double2>(), ncell, dt, Bext, WF2.template as<double2>());
 		if (!r) r = fourier2phys(WF2.template as<double2>(), 4 * ncell, Wt2.template as<double>());
+		if (!r && momentMask) r = moment_correct(Wt1.template as<double>(), Wt2.template as<double>(), ncell, L, momentMask);
 		if (!r) r = transport(Wt2.template as<double>(), ncell, L, h, 0.5 * dt, out);
 		return r;
 	}
@@ -778,6 +786,20 @@ int qbf_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L,
 {
 	if (!e || !W || !out) return set_err(-1, "null argument");
 	return e->inhom_step(W, ncell, L, h, dt, Bext, out);
+}
+
+int qbf_set_moment_correction(qbf_engine* e, int mask)
+{
+	if (!e || mask < 0 || mask > 7) return set_err(-1, "bad argument");
+	e->momentMask = mask;
+	return 0;
+}
+
+int qbf_moment_correct(qbf_engine* e, const double* Wold, double* Wnew, int ncell, double L, int mask)
+{
+	if (!e || !Wold || !Wnew || ncell < 1 || mask < 0 || mask > 7) return set_err(-1, "bad argument");
+	if (e->moment_correct(Wold, Wnew, ncell, L, mask)) return set_err(-2, "k_moment_correct launch failed");
+	return 0;
 }
 
 int qbf_transport_line_periodic(qbf_engine* e, const double* U, int n, double A, double h, double dt, double* out)

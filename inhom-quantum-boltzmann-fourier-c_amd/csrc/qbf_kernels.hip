@@ -450,6 +450,100 @@ __global__ __launch_bounds__(256) void k_transport_line(const double* __restrict
 		- S * (bp1 - bm1) + Sabs * (bp1 - 2 * b0 + bm1));
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP64 conservation correction of a collision step, one workgroup per cell.
+// dW = Wnew - Wold (physical).  The trace part is corrected by
+//   dW0 -= |w0| (lambda . psi),  psi = (1, v_x, v_y, |v|^2/2),
+// with lambda solving G lambda = m, G_jk = sum |w0| psi_j psi_k, m_j = sum psi_j dW0 (weighted
+// least squares: the smallest |w0|^-1-weighted change that zeroes the selected moments of dW0);
+// each spin component by dW_c -= (sum dW_c / sum |w0|) |w0|.  mask bit 0: mass and spin,
+// bit 1: momentum, bit 2: energy (moments as in visualization/wignerMoments.m:15-37).
+__device__ __forceinline__ double block_sum_256(double x, double* sh)
+{
+	for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+	const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+	__syncthreads();
+	if (l == 0) sh[w] = x;
+	__syncthreads();
+	double r = 0;
+	for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
+	return r;
+}
+
+__global__ __launch_bounds__(256) void k_moment_correct(const double* __restrict__ Wold, double* __restrict__ Wnew,
+	int N, double L, int mask)
+{
+	__shared__ double sh[4];
+	__shared__ double sol[8];
+	const int NN = N * N;
+	const size_t cb = (size_t)blockIdx.x * 4 * NN;
+	const double sc = (2 * L) / N;
+	// accumulate the Gram matrix (10 entries, [0] = sum |w0|), the trace moments (4), spin sums (3)
+	double acc[17];
+	for (int i = 0; i < 17; i++) acc[i] = 0;
+	for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+		const int l = e / N, k = e % N;
+		const double vx = l < N / 2 ? sc * l : sc * (l - N);
+		const double vy = k < N / 2 ? sc * k : sc * (k - N);
+		const double ps[4] = {1.0, vx, vy, 0.5 * (vx * vx + vy * vy)};
+		const double w = fabs(Wold[cb + e]);
+		const double d0 = Wnew[cb + e] - Wold[cb + e];
+		int t = 0;
+		for (int a = 0; a < 4; a++)
+			for (int b = a; b < 4; b++) acc[t++] += w * ps[a] * ps[b];
+		for (int a = 0; a < 4; a++) acc[10 + a] += ps[a] * d0;
+		for (int c = 1; c < 4; c++) acc[13 + c] += Wnew[cb + (size_t)c * NN + e] - Wold[cb + (size_t)c * NN + e];
+	}
+	double tot[17];
+	for (int i = 0; i < 17; i++) tot[i] = block_sum_256(acc[i], sh);
+	if (threadIdx.x == 0) {
+		// selected constraints
+		int idx[4], n = 0;
+		if (mask & 1) idx[n++] = 0;
+		if (mask & 2) { idx[n++] = 1; idx[n++] = 2; }
+		if (mask & 4) idx[n++] = 3;
+		double G[4][4], m[4], lam[4] = {0, 0, 0, 0};
+		auto gram = [&](int a, int b) {
+			if (a > b) { int t = a; a = b; b = t; }
+			const int off[4] = {0, 4, 7, 9};
+			return tot[off[a] + (b - a)];
+		};
+		for (int i = 0; i < n; i++) {
+			for (int j = 0; j < n; j++) G[i][j] = gram(idx[i], idx[j]);
+			m[i] = tot[10 + idx[i]];
+		}
+		// Gaussian elimination with partial pivoting on the n x n system
+		for (int c = 0; c < n; c++) {
+			int pv = c;
+			for (int r = c + 1; r < n; r++) if (fabs(G[r][c]) > fabs(G[pv][c])) pv = r;
+			for (int j = 0; j < n; j++) { double t = G[c][j]; G[c][j] = G[pv][j]; G[pv][j] = t; }
+			{ double t = m[c]; m[c] = m[pv]; m[pv] = t; }
+			for (int r = c + 1; r < n; r++) {
+				const double f = G[c][c] != 0 ? G[r][c] / G[c][c] : 0;
+				for (int j = c; j < n; j++) G[r][j] -= f * G[c][j];
+				m[r] -= f * m[c];
+			}
+		}
+		for (int c = n - 1; c >= 0; c--) {
+			double t = m[c];
+			for (int j = c + 1; j < n; j++) t -= G[c][j] * lam[idx[j]];
+			lam[idx[c]] = G[c][c] != 0 ? t / G[c][c] : 0;
+		}
+		for (int i = 0; i < 4; i++) sol[i] = lam[i];
+		const double wsum = tot[0];
+		for (int c = 1; c < 4; c++) sol[3 + c] = ((mask & 1) && wsum != 0) ? tot[13 + c] / wsum : 0;
+	}
+	__syncthreads();
+	for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+		const int l = e / N, k = e % N;
+		const double vx = l < N / 2 ? sc * l : sc * (l - N);
+		const double vy = k < N / 2 ? sc * k : sc * (k - N);
+		const double w = fabs(Wold[cb + e]);
+		Wnew[cb + e] -= w * (sol[0] + sol[1] * vx + sol[2] * vy + sol[3] * 0.5 * (vx * vx + vy * vy));
+		for (int c = 1; c < 4; c++) Wnew[cb + (size_t)c * NN + e] -= sol[3 + c] * w;
+	}
+}
+
 // y = a * x + b * y  (elementwise, double)
 __global__ void k_axpby(const double* __restrict__ x, double* __restrict__ y, double a, double b, long long n)
 {

@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libqbf.so")
 FP64, FP32 = 0, 1
 CC, CD, CFULL = 1, 2, 3
 TAB_I1, TAB_I4 = 0, 1
+CONS_MASS, CONS_MOMENTUM, CONS_ENERGY = 1, 2, 4
+CONS_ALL = 7
 
 _lib = None
 _dp = C.POINTER(C.c_double)
@@ -74,6 +76,8 @@ def lib():
             "qbf_fourier_to_phys": (i, [vp, vp, i, vp]),
             "qbf_transport_periodic": (i, [vp, vp, i, d, d, d, vp]),
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
+            "qbf_set_moment_correction": (i, [vp, i]),
+            "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
             "qbf_profile": (i, [vp, i]),
             "qbf_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]),
             "qbf_legendre_gauss": (None, [u, d, d, _dp, _dp]),
@@ -163,6 +167,15 @@ class Engine:
         ms, n, k = C.c_double(), C.c_int(), C.c_double()
         _check(lib().qbf_profile_read(self._h, C.byref(ms), C.byref(n), C.byref(k)))
         return ms.value, n.value, k.value
+
+    # --------------------------------------------------------------- conservation correction
+    def set_moment_correction(self, mask: int):
+        _check(lib().qbf_set_moment_correction(self._h, mask))
+
+    def moment_correct(self, Wold, Wnew, L, mask=CONS_ALL):
+        self._bind()
+        _check(lib().qbf_moment_correct(self._h, _ptr(Wold), _ptr(Wnew), Wold.shape[0], L, mask))
+        return Wnew
 
     # --------------------------------------------------------------- stream
     def _bind(self):
