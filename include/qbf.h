@@ -110,6 +110,27 @@ void qbf_fourier_I2(int N, unsigned J, double L, double R, double *psiR1, double
 void qbf_fourier_I3(int N, unsigned J, double L, double R, unsigned M, double *psiR1, double *psiR2);
 void qbf_fourier_I4(int N, unsigned J, double L, double R, double *psiR1, double *psiR2);
 
+/* ---- CNN surrogate of the collision operator (no counterpart in the reference) ----
+ * Six periodic 3x3 convolutions 4 -> 64 -> 64 -> 64 -> 64 -> 64 -> 4 on 64 x 64 grids, GELU
+ * between them, bf16 weights and activations, fp32 accumulation on the matrix cores.
+ * Inputs / outputs: planar fp32 [B][4][64][64] device arrays.  Parameters (host, fp32):
+ * per layer weights [9][cout_pad][cin_pad] then bias [cout_pad]; cin_pad = 32 for layer 0
+ * (4 used), cout_pad = 32 for layer 5 (4 used), 64 otherwise; unused entries are zero. */
+typedef struct qbf_cnn qbf_cnn;
+const char *qbf_cnn_last_error(void);
+int    qbf_cnn_create(int N, int maxBatch, qbf_cnn **out);
+void   qbf_cnn_destroy(qbf_cnn *c);
+int    qbf_cnn_set_stream(qbf_cnn *c, void *hip_stream);
+size_t qbf_cnn_param_count(void);
+int    qbf_cnn_set_params(qbf_cnn *c, const float *host);
+int    qbf_cnn_get_params(qbf_cnn *c, float *host, int grads);
+/* out = oscale * net(iscale * in) */
+int    qbf_cnn_forward(qbf_cnn *c, const float *dIn, int B, float iscale, float *dOut, float oscale);
+/* forward + backward of loss = mean((out - target)^2); gradients stay on the device */
+int    qbf_cnn_gradients(qbf_cnn *c, const float *dIn, const float *dTarget, int B, float iscale, float oscale,
+	float *dOut, float *hostLoss);
+int    qbf_cnn_adam(qbf_cnn *c, float lr, float b1, float b2, float eps);
+
 /* device memory helpers for host code that does not include HIP headers */
 void *qbf_malloc(size_t bytes);
 void  qbf_free(void *p);

@@ -1,0 +1,471 @@
+// qbf_cnn.hip -- periodic 3x3 CNN surrogate of the collision operator: forward, backward and
+// Adam kernels for gfx950 (no counterpart in the reference).
+//
+// Network: six 3x3 convolutions on the S x S velocity grid (S = N), periodic padding, GELU
+// after the first five: 4 -> 64 -> 64 -> 64 -> 64 -> 64 -> 4 channels.  Activations are
+// stored NHWC in bf16 (channels padded: 4 -> 32 at the input, 4 -> 32 at the output), weights
+// as bf16 copies of fp32 master weights, and every convolution accumulates in fp32.
+//
+// Forward and backward-data convolutions are implicit GEMMs on the matrix cores
+// (v_mfma_f32_16x16x32_bf16): a workgroup of 4 waves owns one image row (S = 64 pixels),
+// each wave 16 pixels times all output channels; K = 9 taps x input channels.
+// MFMA operand layout (16 x 16 x 32): lane l holds A[l%16][8*(l/16) .. +7],
+// B[8*(l/16) .. +7][l%16], and the result D[4*(l/16) + r][l%16], r < 4.
+// The weight gradient (a reduction over all pixels) runs on the VALU in fp32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qbf.h"
+
+namespace qbfcnn {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __bf16 bf16;
+
+constexpr int S = 64;           // image side (velocity grid N)
+constexpr int NL = 6;           // layers
+constexpr int CH = 64;          // hidden channels
+constexpr int CIO = 32;         // padded input / output channel count
+constexpr int CREAL = 4;        // real input / output channels (Pauli components)
+
+__host__ __device__ constexpr int cin_of(int l) { return l == 0 ? CIO : CH; }
+__host__ __device__ constexpr int cout_of(int l) { return l == NL - 1 ? CIO : CH; }
+
+__device__ __forceinline__ float gelu(float z)
+{
+	const float c = 0.7978845608028654f;   // sqrt(2/pi)
+	const float u = c * (z + 0.044715f * z * z * z);
+	return 0.5f * z * (1.0f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float z)
+{
+	const float c = 0.7978845608028654f;
+	const float u = c * (z + 0.044715f * z * z * z);
+	const float t = tanhf(u);
+	return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+// planar fp32 [B][4][S][S] -> NHWC bf16 [B][S][S][CIO] (channels >= 4 zero)
+__global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, int B, float scale)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long n = (long long)B * S * S;
+	if (i >= n) return;
+	const int b = (int)(i / (S * S)), p = (int)(i % (S * S));
+	bf16* o = out + i * CIO;
+	for (int c = 0; c < CIO; c++) o[c] = (bf16)(c < CREAL ? scale * x[((size_t)b * CREAL + c) * (S * S) + p] : 0.0f);
+}
+
+// Implicit-GEMM 3x3 periodic convolution on the matrix cores.
+//   out[b][y][x][n] = sum_t sum_k in[b][y + DIR*(ky-1)][x + DIR*(kx-1)][k] * Wt[t][n][k]
+// DIR = +1: forward (Wt = W[t][cout][cin]); DIR = -1: backward data (Wt = W^T[t][cin][cout]).
+// MODE 0: out = bias + value, store pre-activation z (fp32) and GELU(z) (bf16 NHWC)
+// MODE 1: out = bias + value, store fp32 planar [B][4][S][S] (first 4 channels)
+// MODE 2: backward data: g = value * gelu'(zprev) -> bf16 NHWC (gradient of the previous z)
+template <int KC, int NC, int DIR, int MODE>
+__global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ Wt,
+	const float* __restrict__ bias, float* __restrict__ zout, bf16* __restrict__ aout, float* __restrict__ yout,
+	const float* __restrict__ zprev, float oscale)
+{
+	constexpr int NT = NC / 16;       // 16-wide output tiles
+	const int row = blockIdx.x;       // b * S + y
+	const int b = row / S, y = row % S;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int i16 = lane & 15, kq = lane >> 4;
+	const int x = wave * 16 + i16;    // pixel of this lane's A row
+	f32x4 acc[NT];
+#pragma unroll
+	for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+	for (int ky = 0; ky < 3; ky++) {
+		const int ys = (y + DIR * (ky - 1) + S) & (S - 1);
+		for (int kx = 0; kx < 3; kx++) {
+			const int xs = (x + DIR * (kx - 1) + S) & (S - 1);
+			const bf16* src = in + (((size_t)b * S + ys) * S + xs) * KC;
+			const bf16* wt = Wt + (size_t)(ky * 3 + kx) * NC * KC;
+#pragma unroll
+			for (int kb = 0; kb < KC / 32; kb++) {
+				const bf16x8 a = *(const bf16x8*)(src + kb * 32 + kq * 8);
+#pragma unroll
+				for (int t = 0; t < NT; t++) {
+					const bf16x8 w = *(const bf16x8*)(wt + (size_t)(t * 16 + i16) * KC + kb * 32 + kq * 8);
+					acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[t], 0, 0, 0);
+				}
+			}
+		}
+	}
+	// lane holds D[4*kq + r][i16]: pixel wave*16 + 4*kq + r, channel t*16 + i16
+#pragma unroll
+	for (int t = 0; t < NT; t++) {
+		const int n = t * 16 + i16;
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int xo = wave * 16 + kq * 4 + r;
+			const size_t pix = ((size_t)b * S + y) * S + xo;
+			const float v = acc[t][r];
+			if (MODE == 0) {
+				const float z = v + bias[n];
+				if (zout) zout[pix * NC + n] = z;
+				aout[pix * NC + n] = (bf16)gelu(z);
+			} else if (MODE == 1) {
+				if (n < CREAL) yout[((size_t)b * CREAL + n) * (S * S) + (size_t)y * S + xo] = oscale * (v + bias[n]);
+			} else {
+				aout[pix * NC + n] = (bf16)(v * gelu_grad(zprev[pix * NC + n]));
+			}
+		}
+	}
+}
+
+// Weight and bias gradients on the VALU:
+//   dW[t][co][ci] += sum_p dz[p][co] * x[p + off_t][ci],  db[co] += sum_p dz[p][co]
+// One workgroup per (image, block of ROWS rows); thread (co, 16-channel ci block) keeps
+// 9 x 16 accumulators; the rows of x (with a one-pixel halo) and dz are staged in LDS.
+template <int CI, int CO>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dz,
+	float* __restrict__ dW, float* __restrict__ db, int rows)
+{
+	constexpr int NCB = CI / 16;                 // ci blocks of 16
+	constexpr int NTH = CO * NCB;                // active threads
+	__shared__ float xs[3][S + 2][CI + 1];
+	__shared__ float zs[S][CO + 1];
+	const int b = blockIdx.x / (S / rows);
+	const int y0 = (blockIdx.x % (S / rows)) * rows;
+	const int tid = threadIdx.x;
+	const int co = tid / NCB, cb = tid % NCB;
+	float acc[9][16];
+#pragma unroll
+	for (int t = 0; t < 9; t++)
+#pragma unroll
+		for (int j = 0; j < 16; j++) acc[t][j] = 0.f;
+	float bacc = 0.f;
+	for (int y = y0; y < y0 + rows; y++) {
+		__syncthreads();
+		for (int e = tid; e < 3 * (S + 2) * CI; e += blockDim.x) {
+			const int r = e / ((S + 2) * CI), rem = e % ((S + 2) * CI);
+			const int xx = rem / CI, c = rem % CI;
+			const int ys = (y + r - 1 + S) & (S - 1), xsrc = (xx - 1 + S) & (S - 1);
+			xs[r][xx][c] = (float)x[(((size_t)b * S + ys) * S + xsrc) * CI + c];
+		}
+		for (int e = tid; e < S * CO; e += blockDim.x) {
+			const int xx = e / CO, c = e % CO;
+			zs[xx][c] = (float)dz[(((size_t)b * S + y) * S + xx) * CO + c];
+		}
+		__syncthreads();
+		if (tid < NTH) {
+			for (int px = 0; px < S; px++) {
+				const float g = zs[px][co];
+				bacc += g;
+#pragma unroll
+				for (int ky = 0; ky < 3; ky++)
+#pragma unroll
+					for (int kx = 0; kx < 3; kx++)
+#pragma unroll
+						for (int j = 0; j < 16; j++) acc[ky * 3 + kx][j] += g * xs[ky][px + kx][cb * 16 + j];
+			}
+		}
+	}
+	if (tid < NTH) {
+#pragma unroll
+		for (int t = 0; t < 9; t++)
+#pragma unroll
+			for (int j = 0; j < 16; j++) atomicAdd(&dW[((size_t)t * CO + co) * CI + cb * 16 + j], acc[t][j]);
+		if (cb == 0) atomicAdd(&db[co], bacc);
+	}
+}
+
+// loss = sum w (pred - target)^2 / n  (mean over elements), gradient -> bf16 NHWC (CIO channels)
+__global__ void k_loss_grad(const float* __restrict__ pred, const float* __restrict__ target, bf16* __restrict__ g,
+	float* __restrict__ loss, int B, float inv_n, float oscale)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long long n = (long long)B * S * S;
+	float part = 0.f;
+	if (i < n) {
+		const int b = (int)(i / (S * S)), p = (int)(i % (S * S));
+		bf16* o = g + i * CIO;
+		for (int c = 0; c < CIO; c++) {
+			float gv = 0.f;
+			if (c < CREAL) {
+				const size_t e = ((size_t)b * CREAL + c) * (S * S) + p;
+				const float d = pred[e] - target[e];
+				part += d * d;
+				gv = 2.0f * d * inv_n * oscale;   // d loss / d (network output before oscale)
+			}
+			o[c] = (bf16)gv;
+		}
+	}
+	for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+	if ((threadIdx.x & 63) == 0) atomicAdd(loss, part * inv_n);
+}
+
+// Adam on fp32 master weights, then bf16 copies W[t][co][ci] and W^T[t][ci][co].
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+	bf16* __restrict__ wb, bf16* __restrict__ wbt, int co_n, int ci_n, int co_real, int ci_real, int n,
+	float lr, float b1, float b2, float eps, float bc1, float bc2, int is_bias)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int co, ci, t;
+	if (is_bias) { co = i; ci = 0; t = 0; }
+	else { t = i / (co_n * ci_n); co = (i / ci_n) % co_n; ci = i % ci_n; }
+	const bool live = co < co_real && (is_bias || ci < ci_real);
+	float wv = w[i];
+	if (live) {
+		const float gi = g[i];
+		const float mi = b1 * m[i] + (1 - b1) * gi;
+		const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+		m[i] = mi;
+		v[i] = vi;
+		wv -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+	} else {
+		wv = 0.f;
+	}
+	w[i] = wv;
+	if (!is_bias) {
+		wb[i] = (bf16)wv;
+		wbt[((size_t)t * ci_n + ci) * co_n + co] = (bf16)wv;
+	}
+}
+
+// bf16 copies of fp32 weights (after an upload)
+__global__ void k_wcopy(const float* __restrict__ w, bf16* __restrict__ wb, bf16* __restrict__ wbt, int co_n, int ci_n, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int t = i / (co_n * ci_n), co = (i / ci_n) % co_n, ci = i % ci_n;
+	wb[i] = (bf16)w[i];
+	wbt[((size_t)t * ci_n + ci) * co_n + co] = (bf16)w[i];
+}
+
+} // namespace qbfcnn
+
+using namespace qbfcnn;
+
+static thread_local std::string g_cnn_err;
+static int cnn_err(int c, const char* m)
+{
+	g_cnn_err = m;
+	return c;
+}
+#define HCHK(x)                                                              \
+	do {                                                                     \
+		hipError_t e_ = (x);                                                 \
+		if (e_ != hipSuccess) return cnn_err(-2, hipGetErrorString(e_));    \
+	} while (0)
+
+struct qbf_cnn {
+	int maxB = 0;
+	hipStream_t stream = nullptr;
+	// per layer: fp32 master weights / grads / Adam moments, bf16 copies
+	float *w[NL] = {}, *gw[NL] = {}, *mw[NL] = {}, *vw[NL] = {};
+	float *bias[NL] = {}, *gb[NL] = {}, *mb[NL] = {}, *vb[NL] = {};
+	bf16 *wb[NL] = {}, *wbt[NL] = {};
+	// activations: x[l] = input of layer l (bf16 NHWC), z[l] = pre-activation (fp32 NHWC), l < NL-1
+	bf16* x[NL] = {};
+	float* z[NL] = {};
+	bf16* g[NL] = {};     // gradient of z[l] (bf16 NHWC), g[NL-1] = gradient of the output
+	float* loss = nullptr;
+	int step = 0;
+	std::vector<void*> allocs;
+	float* alloc_f(size_t n)
+	{
+		void* p = nullptr;
+		if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) return nullptr;
+		hipMemset(p, 0, n * sizeof(float));
+		allocs.push_back(p);
+		return (float*)p;
+	}
+	bf16* alloc_b(size_t n)
+	{
+		void* p = nullptr;
+		if (hipMalloc(&p, n * sizeof(bf16)) != hipSuccess) return nullptr;
+		hipMemset(p, 0, n * sizeof(bf16));
+		allocs.push_back(p);
+		return (bf16*)p;
+	}
+	~qbf_cnn()
+	{
+		for (void* p : allocs) hipFree(p);
+	}
+};
+
+static size_t wsize(int l) { return (size_t)9 * cout_of(l) * cin_of(l); }
+
+extern "C" {
+
+const char* qbf_cnn_last_error(void) { return g_cnn_err.c_str(); }
+
+int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
+{
+	if (!out || N != S || maxBatch < 1) return cnn_err(-1, "qbf_cnn_create: N must be 64 and maxBatch >= 1");
+	qbf_cnn* c = new qbf_cnn();
+	c->maxB = maxBatch;
+	const size_t px = (size_t)maxBatch * S * S;
+	for (int l = 0; l < NL; l++) {
+		const size_t nw = wsize(l), nb = cout_of(l);
+		c->w[l] = c->alloc_f(nw); c->gw[l] = c->alloc_f(nw); c->mw[l] = c->alloc_f(nw); c->vw[l] = c->alloc_f(nw);
+		c->bias[l] = c->alloc_f(nb); c->gb[l] = c->alloc_f(nb); c->mb[l] = c->alloc_f(nb); c->vb[l] = c->alloc_f(nb);
+		c->wb[l] = c->alloc_b(nw); c->wbt[l] = c->alloc_b(nw);
+		c->x[l] = c->alloc_b(px * cin_of(l));
+		c->g[l] = c->alloc_b(px * cout_of(l));
+		if (l < NL - 1) c->z[l] = c->alloc_f(px * cout_of(l));
+		if (!c->w[l] || !c->gw[l] || !c->mw[l] || !c->vw[l] || !c->bias[l] || !c->gb[l] || !c->mb[l] || !c->vb[l] ||
+			!c->wb[l] || !c->wbt[l] || !c->x[l] || !c->g[l] || (l < NL - 1 && !c->z[l])) {
+			delete c;
+			return cnn_err(-3, "qbf_cnn_create: out of device memory");
+		}
+	}
+	c->loss = c->alloc_f(1);
+	if (hipStreamCreate(&c->stream) != hipSuccess) {
+		delete c;
+		return cnn_err(-2, "hipStreamCreate failed");
+	}
+	*out = c;
+	return 0;
+}
+
+void qbf_cnn_destroy(qbf_cnn* c)
+{
+	if (!c) return;
+	hipStreamSynchronize(c->stream);
+	hipStreamDestroy(c->stream);
+	delete c;
+}
+
+int qbf_cnn_set_stream(qbf_cnn* c, void* s)
+{
+	if (!c || !s) return cnn_err(-1, "bad argument");
+	c->stream = (hipStream_t)s;
+	return 0;
+}
+
+// parameter layout of the host buffer: for each layer, weights [9][cout_pad][cin_pad] then bias [cout_pad]
+size_t qbf_cnn_param_count(void)
+{
+	size_t n = 0;
+	for (int l = 0; l < NL; l++) n += wsize(l) + cout_of(l);
+	return n;
+}
+
+int qbf_cnn_set_params(qbf_cnn* c, const float* host)
+{
+	if (!c || !host) return cnn_err(-1, "bad argument");
+	size_t o = 0;
+	for (int l = 0; l < NL; l++) {
+		HCHK(hipMemcpy(c->w[l], host + o, wsize(l) * sizeof(float), hipMemcpyHostToDevice));
+		o += wsize(l);
+		HCHK(hipMemcpy(c->bias[l], host + o, cout_of(l) * sizeof(float), hipMemcpyHostToDevice));
+		o += cout_of(l);
+		const int n = (int)wsize(l);
+		hipLaunchKernelGGL(k_wcopy, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->w[l], c->wb[l], c->wbt[l],
+			cout_of(l), cin_of(l), n);
+		HCHK(hipGetLastError());
+		HCHK(hipMemsetAsync(c->mw[l], 0, wsize(l) * sizeof(float), c->stream));
+		HCHK(hipMemsetAsync(c->vw[l], 0, wsize(l) * sizeof(float), c->stream));
+		HCHK(hipMemsetAsync(c->mb[l], 0, cout_of(l) * sizeof(float), c->stream));
+		HCHK(hipMemsetAsync(c->vb[l], 0, cout_of(l) * sizeof(float), c->stream));
+	}
+	c->step = 0;
+	HCHK(hipStreamSynchronize(c->stream));
+	return 0;
+}
+
+int qbf_cnn_get_params(qbf_cnn* c, float* host, int grads)
+{
+	if (!c || !host) return cnn_err(-1, "bad argument");
+	HCHK(hipStreamSynchronize(c->stream));
+	size_t o = 0;
+	for (int l = 0; l < NL; l++) {
+		HCHK(hipMemcpy(host + o, grads ? c->gw[l] : c->w[l], wsize(l) * sizeof(float), hipMemcpyDeviceToHost));
+		o += wsize(l);
+		HCHK(hipMemcpy(host + o, grads ? c->gb[l] : c->bias[l], cout_of(l) * sizeof(float), hipMemcpyDeviceToHost));
+		o += cout_of(l);
+	}
+	return 0;
+}
+
+static int run_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale, bool train)
+{
+	const long long npx = (long long)B * S * S;
+	hipLaunchKernelGGL(k_to_nhwc, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B, iscale);
+	const dim3 grid(B * S), blk(256);
+	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
+		train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f);
+	for (int l = 1; l < NL - 1; l++)
+		hipLaunchKernelGGL((k_conv_mfma<CH, CH, 1, 0>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
+			train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f);
+	hipLaunchKernelGGL((k_conv_mfma<CH, CIO, 1, 1>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
+		c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale);
+	HCHK(hipGetLastError());
+	return 0;
+}
+
+// out = oscale * net(iscale * in), planar fp32 [B][4][64][64] device arrays
+int qbf_cnn_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale)
+{
+	if (!c || !dIn || !dOut || B < 1 || B > c->maxB) return cnn_err(-1, "qbf_cnn_forward: bad argument");
+	return run_forward(c, dIn, B, iscale, dOut, oscale, false);
+}
+
+// gradients of loss = mean (out - target)^2 into the gradient buffers (no update); loss -> *hostLoss
+int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B, float iscale, float oscale,
+	float* dOut, float* hostLoss)
+{
+	if (!c || !dIn || !dTarget || !dOut || B < 1 || B > c->maxB) return cnn_err(-1, "qbf_cnn_gradients: bad argument");
+	int r = run_forward(c, dIn, B, iscale, dOut, oscale, true);
+	if (r) return r;
+	for (int l = 0; l < NL; l++) {
+		HCHK(hipMemsetAsync(c->gw[l], 0, wsize(l) * sizeof(float), c->stream));
+		HCHK(hipMemsetAsync(c->gb[l], 0, cout_of(l) * sizeof(float), c->stream));
+	}
+	HCHK(hipMemsetAsync(c->loss, 0, sizeof(float), c->stream));
+	const long long npx = (long long)B * S * S;
+	const float inv_n = 1.0f / (float)(npx * CREAL);
+	hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dOut, dTarget,
+		c->g[NL - 1], c->loss, B, inv_n, oscale);
+	const dim3 grid(B * S), blk(256);
+	const int rows = 16;
+	const dim3 wgrid(B * (S / rows));
+	// last layer: weight gradient, then data gradient into the last hidden pre-activation
+	hipLaunchKernelGGL((k_conv_wgrad<CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gw[NL - 1],
+		c->gb[NL - 1], rows);
+	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, -1, 2>), grid, blk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
+		nullptr, c->g[NL - 2], nullptr, c->z[NL - 2], 1.0f);
+	for (int l = NL - 2; l >= 1; l--) {
+		hipLaunchKernelGGL((k_conv_wgrad<CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows);
+		hipLaunchKernelGGL((k_conv_mfma<CH, CH, -1, 2>), grid, blk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
+			c->g[l - 1], nullptr, c->z[l - 1], 1.0f);
+	}
+	hipLaunchKernelGGL((k_conv_wgrad<CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows);
+	HCHK(hipGetLastError());
+	if (hostLoss) {
+		HCHK(hipMemcpyAsync(hostLoss, c->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+		HCHK(hipStreamSynchronize(c->stream));
+	}
+	return 0;
+}
+
+// one Adam step with the gradients of qbf_cnn_gradients
+int qbf_cnn_adam(qbf_cnn* c, float lr, float b1, float b2, float eps)
+{
+	if (!c) return cnn_err(-1, "null cnn");
+	c->step++;
+	const float bc1 = 1.0f - powf(b1, (float)c->step), bc2 = 1.0f - powf(b2, (float)c->step);
+	for (int l = 0; l < NL; l++) {
+		const int co = cout_of(l), ci = cin_of(l);
+		const int co_r = l == NL - 1 ? CREAL : CH, ci_r = l == 0 ? CREAL : CH;
+		const int n = (int)wsize(l);
+		hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->w[l], c->mw[l], c->vw[l], c->gw[l],
+			c->wb[l], c->wbt[l], co, ci, co_r, ci_r, n, lr, b1, b2, eps, bc1, bc2, 0);
+		hipLaunchKernelGGL(k_adam, dim3((co + 255) / 256), dim3(256), 0, c->stream, c->bias[l], c->mb[l], c->vb[l],
+			c->gb[l], nullptr, nullptr, co, 1, co_r, 1, co, lr, b1, b2, eps, bc1, bc2, 1);
+	}
+	HCHK(hipGetLastError());
+	return 0;
+}
+
+} // extern "C"
