@@ -51,6 +51,7 @@ int  qbf_engine_precision(const qbf_engine *e);
 /* run on an existing stream (e.g. torch.cuda.current_stream().cuda_stream); NULL = own stream */
 int  qbf_set_stream(qbf_engine *e, void *hip_stream);
 int  qbf_sync(qbf_engine *e);
+void *qbf_get_stream(qbf_engine *e);     /* the stream the engine launches on */
 
 /* upload all four tables (host memory) */
 int qbf_set_quadrature(qbf_engine *e, const qbf_quad_tables *q);
@@ -130,6 +131,18 @@ int    qbf_cnn_forward(qbf_cnn *c, const float *dIn, int B, float iscale, float 
 int    qbf_cnn_gradients(qbf_cnn *c, const float *dIn, const float *dTarget, int B, float iscale, float oscale,
 	float *dOut, float *hostLoss);
 int    qbf_cnn_adam(qbf_cnn *c, float lr, float b1, float b2, float eps);
+/* Midpoint collision step with C(W) ~ oscale * net(iscale * W) on physical states
+ * [ncell][4][64][64] (device doubles), W_half = W + dt/2 C(W), out = W + dt C(W_half), then the
+ * FP64 moment correction of out - W (mask QBF_CONS_*, 0 = none).  Launches on e's stream. */
+int    qbf_surrogate_collision_step(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double dt,
+	float iscale, float oscale, double L, int mask, double *dOut);
+/* *residual = sum|C_net - C| / sum|C| over the cells hostIdx[0..nsample-1] of W (physical, device),
+ * C = Cc + Cd computed exactly by e; nsample <= 64.  Synchronous. */
+int    qbf_surrogate_residual(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, const int *hostIdx,
+	int nsample, float iscale, float oscale, double *residual);
+/* Strang step as qbf_inhom_step_periodic with the surrogate collision step; ncell <= maxBatch. */
+int    qbf_surrogate_inhom_step_periodic(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double L,
+	double h, double dt, float iscale, float oscale, int mask, double *dOut);
 
 /* device memory helpers for host code that does not include HIP headers */
 void *qbf_malloc(size_t bytes);

@@ -241,6 +241,69 @@ __global__ void k_wcopy(const float* __restrict__ w, bf16* __restrict__ wb, bf16
 	wbt[((size_t)t * ci_n + ci) * co_n + co] = (bf16)w[i];
 }
 
+// surrogate midpoint step, elementwise over the [ncell][4][S][S] state; products and sums are
+// rounded separately (no fused multiply-add), so the step equals its composition from single
+// network evaluations bit for bit
+__global__ void k_to_f32(const double* __restrict__ W, float* __restrict__ x, long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) x[i] = (float)W[i];
+}
+// x = (float)(W + a C)
+__global__ void k_half_f32(const double* __restrict__ W, const float* __restrict__ Cv, double a, float* __restrict__ x,
+	long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) x[i] = (float)__dadd_rn(W[i], __dmul_rn(a, (double)Cv[i]));
+}
+// out = W + a C
+__global__ void k_update_f64(const double* __restrict__ W, const float* __restrict__ Cv, double a, double* __restrict__ out,
+	long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = __dadd_rn(W[i], __dmul_rn(a, (double)Cv[i]));
+}
+
+// gather cells idx[j] of W [*][4][S][S] into out [nsel][4][S][S]
+__global__ void k_gather_cells(const double* __restrict__ W, const int* __restrict__ idx, double* __restrict__ out, int nsel)
+{
+	constexpr int CELL = CREAL * S * S;
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (long long)nsel * CELL) return;
+	const int j = (int)(i / CELL), e = (int)(i % CELL);
+	out[i] = W[(size_t)idx[j] * CELL + e];
+}
+
+// res[0] = sum |a - b|, res[1] = sum |b|  (a fp32, b fp64; one workgroup of 1024 threads)
+__global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ a, const double* __restrict__ b,
+	long long n, double* __restrict__ res)
+{
+	__shared__ double sh[2][16];
+	double d = 0, m = 0;
+	for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+		d += fabs((double)a[i] - b[i]);
+		m += fabs(b[i]);
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		d += __shfl_xor(d, o);
+		m += __shfl_xor(m, o);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		sh[0][threadIdx.x >> 6] = d;
+		sh[1][threadIdx.x >> 6] = m;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double D = 0, M = 0;
+		for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+			D += sh[0][w];
+			M += sh[1][w];
+		}
+		res[0] = D;
+		res[1] = M;
+	}
+}
+
 } // namespace qbfcnn
 
 using namespace qbfcnn;
@@ -259,7 +322,8 @@ static int cnn_err(int c, const char* m)
 
 struct qbf_cnn {
 	int maxB = 0;
-	hipStream_t stream = nullptr;
+	hipStream_t own = nullptr;       // created by qbf_cnn_create, destroyed by qbf_cnn_destroy
+	hipStream_t stream = nullptr;    // where work is launched (own, or the caller's)
 	// per layer: fp32 master weights / grads / Adam moments, bf16 copies
 	float *w[NL] = {}, *gw[NL] = {}, *mw[NL] = {}, *vw[NL] = {};
 	float *bias[NL] = {}, *gb[NL] = {}, *mb[NL] = {}, *vb[NL] = {};
@@ -269,6 +333,13 @@ struct qbf_cnn {
 	float* z[NL] = {};
 	bf16* g[NL] = {};     // gradient of z[l] (bf16 NHWC), g[NL-1] = gradient of the output
 	float* loss = nullptr;
+	float *sx = nullptr, *sc = nullptr;   // surrogate step: network input / output, fp32 [maxB][4][S][S]
+	double* sw = nullptr;                 // surrogate inhom step: state after the first half transport
+	double* sw2 = nullptr;                // ... and after the collision step
+	static constexpr int RMAX = 64;       // residual check: at most RMAX sampled cells
+	int* sidx = nullptr;                  // sampled cell indices (device)
+	double *swf = nullptr, *swc = nullptr; // Fourier state / exact collision of the sample (complex)
+	double* sres = nullptr;               // two partial sums of the residual
 	int step = 0;
 	std::vector<void*> allocs;
 	float* alloc_f(size_t n)
@@ -320,10 +391,38 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		}
 	}
 	c->loss = c->alloc_f(1);
-	if (hipStreamCreate(&c->stream) != hipSuccess) {
+	c->sx = c->alloc_f(px * CREAL);
+	c->sc = c->alloc_f(px * CREAL);
+	{
+		void *p1 = nullptr, *p2 = nullptr;
+		if (hipMalloc(&p1, px * CREAL * sizeof(double)) == hipSuccess) c->allocs.push_back(p1);
+		else p1 = nullptr;
+		if (hipMalloc(&p2, px * CREAL * sizeof(double)) == hipSuccess) c->allocs.push_back(p2);
+		else p2 = nullptr;
+		c->sw = (double*)p1;
+		c->sw2 = (double*)p2;
+	}
+	{
+		void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr;
+		const size_t cplx = (size_t)qbf_cnn::RMAX * CREAL * S * S * 2 * sizeof(double);
+		if (hipMalloc(&p1, qbf_cnn::RMAX * sizeof(int)) == hipSuccess) c->allocs.push_back(p1); else p1 = nullptr;
+		if (hipMalloc(&p2, cplx) == hipSuccess) c->allocs.push_back(p2); else p2 = nullptr;
+		if (hipMalloc(&p3, cplx) == hipSuccess) c->allocs.push_back(p3); else p3 = nullptr;
+		if (hipMalloc(&p4, 2 * sizeof(double)) == hipSuccess) c->allocs.push_back(p4); else p4 = nullptr;
+		c->sidx = (int*)p1;
+		c->swf = (double*)p2;
+		c->swc = (double*)p3;
+		c->sres = (double*)p4;
+	}
+	if (!c->loss || !c->sx || !c->sc || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
+		delete c;
+		return cnn_err(-3, "qbf_cnn_create: out of device memory");
+	}
+	if (hipStreamCreate(&c->own) != hipSuccess) {
 		delete c;
 		return cnn_err(-2, "hipStreamCreate failed");
 	}
+	c->stream = c->own;
 	*out = c;
 	return 0;
 }
@@ -332,14 +431,14 @@ void qbf_cnn_destroy(qbf_cnn* c)
 {
 	if (!c) return;
 	hipStreamSynchronize(c->stream);
-	hipStreamDestroy(c->stream);
+	hipStreamDestroy(c->own);
 	delete c;
 }
 
 int qbf_cnn_set_stream(qbf_cnn* c, void* s)
 {
-	if (!c || !s) return cnn_err(-1, "bad argument");
-	c->stream = (hipStream_t)s;
+	if (!c) return cnn_err(-1, "bad argument");
+	c->stream = s ? (hipStream_t)s : c->own;
 	return 0;
 }
 
@@ -465,6 +564,97 @@ int qbf_cnn_adam(qbf_cnn* c, float lr, float b1, float b2, float eps)
 			c->gb[l], nullptr, nullptr, co, 1, co_r, 1, co, lr, b1, b2, eps, bc1, bc2, 1);
 	}
 	HCHK(hipGetLastError());
+	return 0;
+}
+
+// Midpoint collision step with the network in place of the collision operator, on physical
+// states [ncell][4][64][64] (double, device):
+//   C(W) ~ oscale * net(iscale * W),  W_half = W + dt/2 C(W),  out = W + dt C(W_half),
+// then the FP64 moment correction of out - W (mask as qbf_moment_correct; 0 = none).
+// Runs on the engine's stream; ncell may exceed maxBatch (processed in chunks).
+int qbf_surrogate_collision_step(qbf_cnn* c, qbf_engine* e, const double* dW, int ncell, double dt, float iscale,
+	float oscale, double L, int mask, double* dOut)
+{
+	if (!c || !e || !dW || !dOut || ncell < 1) return cnn_err(-1, "qbf_surrogate_collision_step: bad argument");
+	if (qbf_engine_N(e) != S) return cnn_err(-1, "qbf_surrogate_collision_step: engine grid must be 64");
+	if (dW == dOut) return cnn_err(-1, "qbf_surrogate_collision_step: in-place update not supported");
+	hipStream_t saved = c->stream;
+	c->stream = (hipStream_t)qbf_get_stream(e);
+	int r = 0;
+	for (int c0 = 0; c0 < ncell && !r; c0 += c->maxB) {
+		const int nb = ncell - c0 < c->maxB ? ncell - c0 : c->maxB;
+		const long long n = (long long)nb * CREAL * S * S;
+		const size_t off = (size_t)c0 * CREAL * S * S;
+		const dim3 g((unsigned)((n + 255) / 256)), b(256);
+		hipLaunchKernelGGL(k_to_f32, g, b, 0, c->stream, dW + off, c->sx, n);
+		r = run_forward(c, c->sx, nb, iscale, c->sc, oscale, false);
+		if (r) break;
+		hipLaunchKernelGGL(k_half_f32, g, b, 0, c->stream, dW + off, c->sc, 0.5 * dt, c->sx, n);
+		r = run_forward(c, c->sx, nb, iscale, c->sc, oscale, false);
+		if (r) break;
+		hipLaunchKernelGGL(k_update_f64, g, b, 0, c->stream, dW + off, c->sc, dt, dOut + off, n);
+		if (hipGetLastError() != hipSuccess) r = cnn_err(-2, "surrogate step launch failed");
+	}
+	if (!r && mask && qbf_moment_correct(e, dW, dOut, ncell, L, mask))
+		r = cnn_err(-2, qbf_last_error());
+	c->stream = saved;
+	return r;
+}
+
+// Residual of the surrogate on sampled cells of physical states W: the exact collision
+// operator C = Cc + Cd of the engine (any precision) against oscale * net(iscale * W),
+//   *residual = sum |C_net - C| / sum |C|  over the nsample cells hostIdx[0..nsample-1].
+// Synchronous (the result is read back to the host).
+int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncell, const int* hostIdx, int nsample,
+	float iscale, float oscale, double* residual)
+{
+	if (!c || !e || !dW || !hostIdx || !residual || nsample < 1 || nsample > qbf_cnn::RMAX || nsample > c->maxB)
+		return cnn_err(-1, "qbf_surrogate_residual: bad argument (1 <= nsample <= 64 and <= maxBatch)");
+	if (qbf_engine_N(e) != S) return cnn_err(-1, "qbf_surrogate_residual: engine grid must be 64");
+	for (int j = 0; j < nsample; j++)
+		if (hostIdx[j] < 0 || hostIdx[j] >= ncell) return cnn_err(-1, "qbf_surrogate_residual: cell index out of range");
+	hipStream_t saved = c->stream;
+	c->stream = (hipStream_t)qbf_get_stream(e);
+	int r = 0;
+	const long long n = (long long)nsample * CREAL * S * S;
+	double h[2] = {0, 0};
+	if (hipMemcpyAsync(c->sidx, hostIdx, nsample * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+		r = cnn_err(-2, "index upload failed");
+	if (!r) {
+		hipLaunchKernelGGL(k_gather_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, dW, c->sidx, c->sw,
+			nsample);
+		if (qbf_phys_to_fourier(e, c->sw, CREAL * nsample, c->swf) ||
+			qbf_collision(e, c->swf, nsample, QBF_C, c->swc, nullptr, nullptr, nullptr, 1.0) ||
+			qbf_fourier_to_phys(e, c->swc, CREAL * nsample, c->sw2))
+			r = cnn_err(-2, qbf_last_error());
+	}
+	if (!r) {
+		hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->sw, c->sx, n);
+		r = run_forward(c, c->sx, nsample, iscale, c->sc, oscale, false);
+	}
+	if (!r) {
+		hipLaunchKernelGGL(k_l1_residual, dim3(1), dim3(1024), 0, c->stream, c->sc, c->sw2, n, c->sres);
+		if (hipMemcpyAsync(h, c->sres, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+			hipStreamSynchronize(c->stream) != hipSuccess)
+			r = cnn_err(-2, "residual read-back failed");
+	}
+	c->stream = saved;
+	if (!r) *residual = h[1] > 0 ? h[0] / h[1] : (h[0] > 0 ? INFINITY : 0.0);
+	return r;
+}
+
+// Strang step of the periodic inhomogeneous problem with the surrogate collision step:
+// half transport (FP64, exact), surrogate midpoint collision step + moment correction, half
+// transport.  ncell <= maxBatch (the intermediate states live in the network's buffers).
+int qbf_surrogate_inhom_step_periodic(qbf_cnn* c, qbf_engine* e, const double* dW, int ncell, double L, double h,
+	double dt, float iscale, float oscale, int mask, double* dOut)
+{
+	if (!c || !e || !dW || !dOut || ncell < 1 || ncell > c->maxB)
+		return cnn_err(-1, "qbf_surrogate_inhom_step_periodic: bad argument (ncell must be in [1, maxBatch])");
+	if (qbf_transport_periodic(e, dW, ncell, L, h, 0.5 * dt, c->sw)) return cnn_err(-2, qbf_last_error());
+	int r = qbf_surrogate_collision_step(c, e, c->sw, ncell, dt, iscale, oscale, L, mask, c->sw2);
+	if (r) return r;
+	if (qbf_transport_periodic(e, c->sw2, ncell, L, h, 0.5 * dt, dOut)) return cnn_err(-2, qbf_last_error());
 	return 0;
 }
 

@@ -657,6 +657,8 @@ void qbf_engine_destroy(qbf_engine* e)
 int qbf_engine_N(const qbf_engine* e) { return e ? e->gridN : 0; }
 int qbf_engine_precision(const qbf_engine* e) { return e ? e->precision : -1; }
 
+void* qbf_get_stream(qbf_engine* e) { return e ? (void*)e->stream : nullptr; }
+
 int qbf_set_stream(qbf_engine* e, void* s)
 {
 	if (!e) return set_err(-1, "null engine");

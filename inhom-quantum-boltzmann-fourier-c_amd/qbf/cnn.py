@@ -1,0 +1,176 @@
+"""ctypes binding of the CNN surrogate kernels (qbf_cnn_* in include/qbf.h).
+
+The network maps a physical Wigner state of one cell, fp32 planar [4, 64, 64], to a
+[4, 64, 64] field (trained to the collision operator in physical space).  All tensors are
+torch float32 cuda tensors of shape [B, 4, 64, 64]; the kernels run on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import QBFError, lib
+
+NL, CH, CIO, CREAL, S = 6, 64, 32, 4, 64
+
+_sig_done = False
+
+
+def _lib():
+    global _sig_done
+    L = lib()
+    if not _sig_done:
+        vp, i, f, sz = C.c_void_p, C.c_int, C.c_float, C.c_size_t
+        fp = C.POINTER(C.c_float)
+        sig = {
+            "qbf_cnn_last_error": (C.c_char_p, []),
+            "qbf_cnn_create": (i, [i, i, C.POINTER(vp)]),
+            "qbf_cnn_destroy": (None, [vp]),
+            "qbf_cnn_set_stream": (i, [vp, vp]),
+            "qbf_cnn_param_count": (sz, []),
+            "qbf_cnn_set_params": (i, [vp, fp]),
+            "qbf_cnn_get_params": (i, [vp, fp, i]),
+            "qbf_cnn_forward": (i, [vp, vp, i, f, vp, f]),
+            "qbf_cnn_gradients": (i, [vp, vp, vp, i, f, f, vp, fp]),
+            "qbf_cnn_adam": (i, [vp, f, f, f, f]),
+            "qbf_surrogate_collision_step": (i, [vp, vp, vp, i, C.c_double, f, f, C.c_double, i, vp]),
+            "qbf_surrogate_residual": (i, [vp, vp, vp, i, C.POINTER(i), i, f, f, C.POINTER(C.c_double)]),
+            "qbf_surrogate_inhom_step_periodic": (i, [vp, vp, vp, i, C.c_double, C.c_double, C.c_double, f, f, i,
+                                                      vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _sig_done = True
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise QBFError(_lib().qbf_cnn_last_error().decode())
+
+
+def _ptr(t):
+    assert t.is_cuda and t.is_contiguous() and t.dtype.is_floating_point
+    return C.c_void_p(t.data_ptr())
+
+
+def param_count() -> int:
+    return int(_lib().qbf_cnn_param_count())
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights on the used channels, zero biases and zero padded entries; the last
+    layer is scaled by 0.1 so that an untrained network predicts small values."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for l in range(NL):
+        ci = CIO if l == 0 else CH
+        co = CIO if l == NL - 1 else CH
+        ci_r = CREAL if l == 0 else CH
+        co_r = CREAL if l == NL - 1 else CH
+        W = np.zeros((9, co, ci), np.float32)
+        W[:, :co_r, :ci_r] = rng.standard_normal((9, co_r, ci_r)) * np.sqrt(2.0 / (9 * ci_r))
+        if l == NL - 1:
+            W *= 0.1
+        out += [W.ravel(), np.zeros(co, np.float32)]
+    return np.concatenate(out)
+
+
+class CNN:
+    """Six-layer periodic 3x3 CNN (bf16 weights/activations, fp32 accumulation) on the GPU."""
+
+    def __init__(self, max_batch: int, params: np.ndarray | None = None, seed: int = 0):
+        import torch
+        if not torch.cuda.is_available():
+            raise QBFError("no HIP device available")
+        self._torch = torch
+        h = C.c_void_p()
+        _check(_lib().qbf_cnn_create(S, max_batch, C.byref(h)))
+        self._h = h
+        self.max_batch = max_batch
+        self.set_params(init_params(seed) if params is None else params)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().qbf_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bind(self):
+        _check(_lib().qbf_cnn_set_stream(self._h, C.c_void_p(self._torch.cuda.current_stream().cuda_stream)))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != param_count():
+            raise QBFError(f"expected {param_count()} parameters, got {p.size}")
+        self._bind()
+        _check(_lib().qbf_cnn_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def get_params(self, grads: bool = False) -> np.ndarray:
+        self._bind()
+        p = np.empty(param_count(), np.float32)
+        _check(_lib().qbf_cnn_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)), 1 if grads else 0))
+        return p
+
+    def forward(self, x, iscale: float = 1.0, oscale: float = 1.0, out=None):
+        """out = oscale * net(iscale * x); x float32 cuda [B, 4, 64, 64]."""
+        self._bind()
+        out = self._torch.empty_like(x) if out is None else out
+        _check(_lib().qbf_cnn_forward(self._h, _ptr(x), x.shape[0], iscale, _ptr(out), oscale))
+        return out
+
+    def gradients(self, x, target, iscale: float = 1.0, oscale: float = 1.0, out=None):
+        """Forward and backward of mean((out - target)^2); returns (loss, out).  The parameter
+        gradients stay on the device (get_params(grads=True) reads them)."""
+        self._bind()
+        out = self._torch.empty_like(x) if out is None else out
+        loss = C.c_float()
+        _check(_lib().qbf_cnn_gradients(self._h, _ptr(x), _ptr(target), x.shape[0], iscale, oscale, _ptr(out),
+                                        C.byref(loss)))
+        return loss.value, out
+
+    def adam(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+        self._bind()
+        _check(_lib().qbf_cnn_adam(self._h, lr, b1, b2, eps))
+
+    # ------------------------------------------------------------------ stepping with the surrogate
+    def collision_step(self, engine, W, dt, iscale, oscale, L, mask=0, out=None):
+        """Midpoint collision step of physical states W (float64 cuda [ncell, 4, 64, 64]) with the
+        network as collision operator, followed by the moment correction selected by mask."""
+        engine._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(_lib().qbf_surrogate_collision_step(self._h, engine._h, _ptrd(W), W.shape[0], dt, iscale, oscale, L,
+                                                   mask, _ptrd(out)))
+        return out
+
+    def residual(self, engine, W, cells, iscale, oscale):
+        """sum|C_net - C| / sum|C| over the given cells of W (float64 cuda [ncell, 4, 64, 64]), with
+        C the engine's exact collision operator."""
+        engine._bind()
+        idx = np.ascontiguousarray(cells, np.int32)
+        r = C.c_double()
+        _check(_lib().qbf_surrogate_residual(self._h, engine._h, _ptrd(W), W.shape[0],
+                                             idx.ctypes.data_as(C.POINTER(C.c_int)), idx.size, iscale, oscale,
+                                             C.byref(r)))
+        return r.value
+
+    def inhom_step_periodic(self, engine, W, L, h, dt, iscale, oscale, mask=0, out=None):
+        """Strang step (half transport, surrogate collision step, half transport)."""
+        engine._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(_lib().qbf_surrogate_inhom_step_periodic(self._h, engine._h, _ptrd(W), W.shape[0], L, h, dt, iscale,
+                                                        oscale, mask, _ptrd(out)))
+        return out
+
+
+def _ptrd(t):
+    assert t.is_cuda and t.is_contiguous() and t.dtype == __import__("torch").float64
+    return C.c_void_p(t.data_ptr())

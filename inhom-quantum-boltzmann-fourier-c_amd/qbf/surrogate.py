@@ -1,0 +1,146 @@
+"""CNN surrogate of the collision operator at N = 64: training data, training on the GPU, and
+the stepping policy that keeps it honest.
+
+Training data are physical states of the periodic workload family (qbf.workloads.periodic)
+at several free-streaming times, with the exact collision operator C = Cc + Cd of the FP32
+engine as target.  The network learns C(W) ~ oscale * net(iscale * W) with Adam on its own
+GPU kernels (qbf.cnn).
+
+Stepping policy (SurrogateStepper.step), per Strang step of the periodic problem:
+  * every `exact_every`-th step is an exact step (FP32 convolution engine, FP64 update);
+  * otherwise a fraction `sample_frac` of the cells (at least one) is checked first: the
+    relative L1 residual of the network against the exact collision operator on those cells;
+    above `tol` the step falls back to the exact step;
+  * a surrogate step is the midpoint collision step with the network, followed by the FP64
+    conservation correction (mass, spin, momentum, energy), between two exact half transports.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import CONS_ALL, FP32, Engine, QBFError, quadrature
+from . import workloads as wl
+from .cnn import CNN
+
+N_SURR = 64
+
+
+def exact_collision_phys(engine, W):
+    """C = Cc + Cd of physical states W (float64 cuda [n, 4, N, N]) in physical space."""
+    WF = engine.phys_to_fourier(W)
+    CF = engine.collision(WF)
+    return engine.fourier_to_phys(CF)
+
+
+def make_engine(N=N_SURR, max_cells=512, J=wl.J_ANG, M=wl.M_GL, precision=FP32):
+    e = Engine(N, precision, max_cells=max_cells)
+    e.set_quadrature(*quadrature(N, J, wl.L_BOX, wl.R_BALL, M))
+    return e
+
+
+def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=256, rng_seed=0):
+    """(X, Y): float32 cuda tensors [n, 4, 64, 64] of states and their exact collision operator."""
+    import torch
+    rng = np.random.default_rng(rng_seed)
+    X, Y = [], []
+    for s in seeds:
+        for t in times:
+            cells = np.sort(rng.choice(Nx, cells_per_state, replace=False))
+            W = wl.periodic(N_SURR, Nx, seed=int(s), t=float(t))[cells]
+            X.append(torch.from_numpy(W).cuda())
+    Xd = torch.cat(X)
+    for i in range(0, Xd.shape[0], chunk):
+        Y.append(exact_collision_phys(engine, Xd[i:i + chunk].contiguous()))
+    Yd = torch.cat(Y)
+    return Xd.float().contiguous(), Yd.float().contiguous()
+
+
+def scales(X, Y):
+    """iscale maps the states to O(1), oscale the collision operator."""
+    return 1.0 / float(X.abs().max()), float(Y.abs().max())
+
+
+def rel_l1(net, X, Y, iscale, oscale, batch=256):
+    num = den = 0.0
+    for i in range(0, X.shape[0], batch):
+        P = net.forward(X[i:i + batch].contiguous(), iscale, oscale)
+        num += float((P - Y[i:i + batch]).abs().sum())
+        den += float(Y[i:i + batch].abs().sum())
+    return num / den
+
+
+def train(net, X, Y, iscale, oscale, steps=3000, batch=32, lr=2e-3, lr_min=2e-5, seed=0, log_every=0):
+    """Adam with a cosine learning-rate schedule on the mean squared error of the normalised
+    target Y / oscale.  Returns the loss history."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    Tn = (Y / oscale).contiguous()
+    n = X.shape[0]
+    hist = []
+    for it in range(steps):
+        idx = torch.randint(0, n, (batch,), generator=g).cuda()
+        xb = X.index_select(0, idx).contiguous()
+        tb = Tn.index_select(0, idx).contiguous()
+        loss, _ = net.gradients(xb, tb, iscale, 1.0)
+        a = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / max(1, steps - 1)))
+        net.adam(a)
+        hist.append(loss)
+        if log_every and it % log_every == 0:
+            print(f"  train step {it}: loss {loss:.3e} lr {a:.2e}", flush=True)
+        if not math.isfinite(loss):
+            raise QBFError(f"training diverged at step {it}")
+    return hist
+
+
+class SurrogateStepper:
+    """Strang steps of the periodic problem with the surrogate collision step and its safeguards."""
+
+    def __init__(self, engine, net, iscale, oscale, L, h, dt, exact_every=10, sample_frac=0.02, tol=1e-3,
+                 mask=CONS_ALL, check=True, seed=0):
+        self.e, self.net = engine, net
+        self.iscale, self.oscale = iscale, oscale
+        self.L, self.h, self.dt = L, h, dt
+        self.exact_every, self.sample_frac, self.tol = exact_every, sample_frac, tol
+        self.mask, self.check = mask, check
+        self.rng = np.random.default_rng(seed)
+        self.nstep = 0
+        self.counts = {"exact": 0, "surrogate": 0, "fallback": 0}
+        self.residuals = []
+        self.engine_mask = mask
+
+    def _exact(self, W, out):
+        self.e.set_moment_correction(self.mask)
+        return self.e.inhom_step_periodic(W, self.L, self.h, self.dt, out=out)
+
+    def step(self, W, out=None):
+        ncell = W.shape[0]
+        kind = "surrogate"
+        if self.exact_every and self.nstep % self.exact_every == 0:
+            kind = "exact"
+        elif self.check and self.sample_frac > 0:
+            ns = min(64, max(1, int(round(self.sample_frac * ncell))))
+            cells = np.sort(self.rng.choice(ncell, ns, replace=False))
+            r = self.net.residual(self.e, W, cells, self.iscale, self.oscale)
+            self.residuals.append(r)
+            if r > self.tol:
+                kind = "fallback"
+        if kind == "surrogate":
+            out = self.net.inhom_step_periodic(self.e, W, self.L, self.h, self.dt, self.iscale, self.oscale,
+                                               self.mask, out=out)
+        else:
+            out = self._exact(W, out)
+        self.counts[kind] += 1
+        self.nstep += 1
+        return out
+
+
+def timed(fn, *a, **k):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = fn(*a, **k)
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0

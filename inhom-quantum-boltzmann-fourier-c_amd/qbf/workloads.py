@@ -25,7 +25,7 @@ def velocity_grid(N: int, L: float = L_BOX):
 
 
 def homogeneous(N: int = 32, seed: int = 0, L: float = L_BOX) -> np.ndarray:
-    """Config 1: sum of two isotropic Gaussians with centres in [-1.5, 1.5]^2, sigma in [0.6, 1.2],
+    """Homogeneous state: sum of two isotropic Gaussians with centres in [-1.5, 1.5]^2, sigma in [0.6, 1.2],
     w0 amplitude in [0.2, 0.5], Pauli direction uniform on the sphere.  Returns [1, 4, N, N]."""
     rng = np.random.default_rng(seed)
     vx, vy = velocity_grid(N, L)
@@ -44,29 +44,39 @@ def homogeneous(N: int = 32, seed: int = 0, L: float = L_BOX) -> np.ndarray:
     return W
 
 
-def periodic(N: int, Nx: int, seed: int = 0, kmax: int = 4, L: float = L_BOX) -> np.ndarray:
-    """Configs 2, 3, 4, 5: cells x_i = (i + 1/2)/Nx on [0, 1).  w0 is a Maxwellian in v whose
-    density and temperature are 1 + 0.3 sin(2 pi x) + sum_{k<=kmax} 0.05 sin(2 pi k x + phase_k)
-    (seeded phases), scaled so that max w0 = 0.5; the Pauli direction rotates about a seeded axis
-    by an integer number of turns over the period.  Returns [Nx, 4, N, N]."""
+def periodic(N: int, Nx: int, seed: int = 0, kmax: int = 4, L: float = L_BOX, t: float = 0.0) -> np.ndarray:
+    """Periodic inhomogeneous state on cells x_i = (i + 1/2)/Nx of [0, 1).  w0 is a Maxwellian in v
+    whose density and temperature are 1 + 0.3 sin(2 pi x) + sum_{k<=kmax} 0.05 sin(2 pi k x + phase_k)
+    (seeded phases), scaled so that max w0 = 0.5 at t = 0; the Pauli direction rotates about a
+    seeded axis by an integer number of turns over the period.  With t > 0 the state is the free
+    streaming of that initial state, W(x, v, t) = W(x - v_x t, v, 0) (transport without
+    collisions), which is how the surrogate's training states leave local equilibrium.
+    Returns [Nx, 4, N, N]."""
     rng = np.random.default_rng(seed)
-    x = (np.arange(Nx) + 0.5) / Nx
     ph_rho = rng.uniform(0, 2 * np.pi, kmax)
     ph_T = rng.uniform(0, 2 * np.pi, kmax)
-    rho = 1 + 0.3 * np.sin(2 * np.pi * x) + sum(0.05 * np.sin(2 * np.pi * (k + 1) * x + ph_rho[k]) for k in range(kmax))
-    T = 1 + 0.3 * np.sin(2 * np.pi * x + 1.0) + sum(0.05 * np.sin(2 * np.pi * (k + 1) * x + ph_T[k]) for k in range(kmax))
     turns = int(rng.integers(1, 4))
     tilt = rng.uniform(0.3, 1.2)
     vx, vy = velocity_grid(N, L)
     v2 = vx ** 2 + vy ** 2
-    w0 = rho[:, None, None] / (2 * np.pi * T[:, None, None]) * np.exp(-v2[None] / (2 * T[:, None, None]))
-    w0 *= 0.5 / w0.max()
-    phi = 2 * np.pi * turns * x
-    n = np.stack([np.sin(tilt) * np.cos(phi), np.sin(tilt) * np.sin(phi), np.full(Nx, np.cos(tilt))], axis=1)
+
+    def profiles(x):
+        rho = 1 + 0.3 * np.sin(2 * np.pi * x) + sum(0.05 * np.sin(2 * np.pi * (k + 1) * x + ph_rho[k]) for k in range(kmax))
+        T = 1 + 0.3 * np.sin(2 * np.pi * x + 1.0) + sum(0.05 * np.sin(2 * np.pi * (k + 1) * x + ph_T[k]) for k in range(kmax))
+        phi = 2 * np.pi * turns * x
+        n = np.stack([np.sin(tilt) * np.cos(phi), np.sin(tilt) * np.sin(phi), np.full(np.shape(x), np.cos(tilt))])
+        return rho / (2 * np.pi * T) * np.exp(-v2 / (2 * T)), n
+
+    x = (np.arange(Nx) + 0.5) / Nx
+    w0_init, _ = profiles(x[:, None, None])
+    scale = 0.5 / w0_init.max()
+    xs = x[:, None, None] - vx[None] * t          # foot of the characteristic, per cell and velocity
+    w0, n = profiles(xs)
+    w0 = w0 * scale
     W = np.empty((Nx, 4, N, N))
     W[:, 0] = w0
     for i in range(3):
-        W[:, i + 1] = 0.8 * w0 * n[:, i, None, None]
+        W[:, i + 1] = 0.8 * w0 * n[i]
     return W
 
 

@@ -1,0 +1,128 @@
+"""CNN surrogate kernels (BF16 MFMA convolutions, VALU weight gradients, Adam) against the
+CPU oracle oracle/cnn_ref.py, which rounds to bfloat16 at the same points.  Tolerances: the
+GPU accumulates in fp32 in a different order, so an activation can round to the neighbouring
+bfloat16 value (relative 2^-8); outputs and gradients agree to 1e-2 relative L2, while a
+transposed MFMA operand or a wrong tap offset gives errors of order 1."""
+import numpy as np
+import pytest
+
+from oracle import cnn_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(seed, B=3, S=64):
+    rng = np.random.default_rng(seed)
+    p = R.init_params(seed)
+    o = 0
+    for l in range(R.NL):
+        o += 9 * R.cout(l) * R.cin(l)
+        nb = 4 if l == R.NL - 1 else 64
+        p[o:o + nb] = 0.05 * rng.standard_normal(nb)
+        o += R.cout(l)
+    # a last layer of ordinary size so that the output is not dominated by the biases
+    o5 = o - R.cout(R.NL - 1) - 9 * R.cout(R.NL - 1) * R.cin(R.NL - 1)
+    p[o5:o5 + 9 * R.cout(R.NL - 1) * R.cin(R.NL - 1)] *= 10
+    x = (0.5 * rng.standard_normal((B, 4, S, S))).astype(np.float32)
+    t = (0.3 * rng.standard_normal((B, 4, S, S))).astype(np.float32)
+    return p.astype(np.float32), x, t
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def test_forward_matches_oracle():
+    import torch
+    from qbf.cnn import CNN
+    p, x, _ = _setup(0)
+    net = CNN(max_batch=4, params=p)
+    out = net.forward(torch.from_numpy(x).cuda(), iscale=1.5, oscale=0.8).cpu().numpy()
+    ref, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=True)
+    ref64, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=False)
+    e = _rel(out, ref)
+    print("forward rel L2 vs bf16 oracle", e, " vs fp64 network", _rel(out, ref64))
+    assert e < 1e-2
+    assert _rel(out, ref64) < 5e-2
+
+
+def test_forward_is_translation_equivariant_and_batch_independent():
+    """Periodic padding: shifting the input grid shifts the output; cells do not interact."""
+    import torch
+    from qbf.cnn import CNN
+    p, x, _ = _setup(1)
+    net = CNN(max_batch=4, params=p)
+    X = torch.from_numpy(x).cuda()
+    y = net.forward(X).cpu().numpy()
+    ys = net.forward(torch.roll(X, shifts=(5, -7), dims=(2, 3)).contiguous()).cpu().numpy()
+    tol = 1e-6 * np.abs(y).max()
+    np.testing.assert_allclose(np.roll(y, shift=(5, -7), axis=(2, 3)), ys, rtol=0, atol=tol)
+    y1 = net.forward(X[1:2].contiguous()).cpu().numpy()
+    np.testing.assert_allclose(y1[0], y[1], rtol=0, atol=tol)
+
+
+def test_gradients_match_oracle():
+    import torch
+    from qbf.cnn import CNN
+    p, x, t = _setup(2, B=2)
+    net = CNN(max_batch=2, params=p)
+    loss, out = net.gradients(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), iscale=1.2, oscale=0.9)
+    g = net.get_params(grads=True)
+    lref, gref, oref = R.gradients(x, t, p, iscale=1.2, oscale=0.9, bf16_round=True)
+    assert abs(loss - lref) <= 5e-3 * lref, (loss, lref)
+    assert _rel(out.cpu().numpy(), oref) < 1e-2
+    o = 0
+    for l in range(R.NL):
+        for n in (9 * R.cout(l) * R.cin(l), R.cout(l)):
+            a, b = g[o:o + n], gref[o:o + n]
+            e = _rel(a, b)
+            print(f"layer {l} size {n}: grad rel L2 {e:.2e}")
+            assert e < 2e-2, (l, n, e)
+            o += n
+    # padded entries get no gradient
+    assert o == g.size
+
+
+def test_adam_step_and_padding():
+    import torch
+    from qbf.cnn import CNN
+    p, x, t = _setup(3, B=2)
+    net = CNN(max_batch=2, params=p)
+    net.gradients(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+    g = net.get_params(grads=True).astype(np.float64)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    net.adam(lr, b1, b2, eps)
+    p1 = net.get_params()
+    m = (1 - b1) * g
+    v = (1 - b2) * g * g
+    want = p - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    live = np.zeros(p.size, bool)
+    o = 0
+    for l in range(R.NL):
+        W = np.zeros((9, R.cout(l), R.cin(l)), bool)
+        W[:, :(4 if l == R.NL - 1 else 64), :(4 if l == 0 else 64)] = True
+        live[o:o + W.size] = W.ravel()
+        o += W.size
+        live[o:o + (4 if l == R.NL - 1 else 64)] = True
+        o += R.cout(l)
+    np.testing.assert_allclose(p1[live], want[live], rtol=1e-5, atol=1e-7)
+    assert np.all(p1[~live] == 0)
+
+
+def test_training_reduces_loss():
+    """Adam on a fixed batch: the loss falls by a large factor within 60 steps."""
+    import torch
+    from qbf.cnn import CNN
+    p, x, _ = _setup(4, B=4)
+    # a smooth target the network can represent: a local nonlinear map of the input
+    t = (0.5 * np.tanh(x) - 0.2 * np.roll(x, 1, axis=2)).astype(np.float32)
+    net = CNN(max_batch=4, params=R.init_params(4))
+    X, T = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    losses = []
+    for it in range(60):
+        loss, _ = net.gradients(X, T)
+        losses.append(loss)
+        net.adam(2e-3)
+    print("loss", losses[0], "->", losses[-1])
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.2 * losses[0]

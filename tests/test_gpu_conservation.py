@@ -41,7 +41,7 @@ def test_selected_moments_vanish_and_change_is_minimal(eng):
             ch = (out - Wnew)[c, 0].ravel()
             A = np.stack([np.abs(Wold[c, 0]).ravel() * psi[j].ravel() for j in sel], axis=1)
             coef, *_ = np.linalg.lstsq(A, ch, rcond=None)
-            assert np.abs(A @ coef - ch).max() < 1e-12 * max(1e-300, np.abs(ch).max())
+            assert np.abs(A @ coef - ch).max() < 1e-10 * max(1e-300, np.abs(ch).max())
 
 
 def test_conserving_increment_is_unchanged_and_mask0_is_identity(eng):

@@ -1,0 +1,116 @@
+"""Surrogate stepping entry points (qbf_surrogate_*) and the stepping policy.
+
+The fused midpoint step is checked against the same step composed from single network
+evaluations; the residual against the engine's exact operator; the policy against exact
+steps (every step exact, or every step falling back, reproduces the exact trajectory bit for
+bit); and the safeguards against the bare surrogate on a briefly trained network."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 64
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    import qbf
+    from qbf import surrogate as S
+    from qbf import workloads as wl
+    from qbf.cnn import CNN
+    e = S.make_engine(max_cells=64, J=8, M=4)
+    net = CNN(max_batch=64, seed=1)
+    W = torch.from_numpy(wl.periodic(N, 64, seed=7, t=0.01)).cuda()
+    return qbf, S, wl, e, net, W, torch
+
+
+def test_fused_collision_step_matches_composition(setup):
+    qbf, S, wl, e, net, W, torch = setup
+    dt, isc, osc = 1e-3, 2.0, 0.05
+    out = net.collision_step(e, W, dt, isc, osc, wl.L_BOX, mask=0)
+    C1 = net.forward(W.float().contiguous(), isc, osc).double()
+    Wh = (W + 0.5 * dt * C1).float().contiguous()
+    C2 = net.forward(Wh, isc, osc).double()
+    ref = W + dt * C2
+    print("fused vs composed max abs diff", (out - ref).abs().max().item())
+    assert torch.equal(out, ref)
+
+
+def test_collision_step_moment_correction(setup):
+    qbf, S, wl, e, net, W, torch = setup
+    dt, isc, osc = 1e-3, 2.0, 0.05
+    out = net.collision_step(e, W, dt, isc, osc, wl.L_BOX, mask=qbf.CONS_ALL).cpu().numpy()
+    raw = net.collision_step(e, W, dt, isc, osc, wl.L_BOX, mask=0).cpu().numpy()
+    w = W.cpu().numpy()
+    m0, m1, mr = wl.moments(w), wl.moments(out), wl.moments(raw)
+    # mass, spin and momentum relative to the largest cell mass, energy to the largest cell energy
+    scale = np.array([np.abs(m0[:, 0]).max()] * 6 + [np.abs(m0[:, 6]).max()])
+    drift = (np.abs(m1 - m0) / scale).max()
+    drift_raw = (np.abs(mr - m0) / scale).max()
+    print("moment drift corrected", drift, "uncorrected", drift_raw)
+    assert drift < 1e-14
+    assert drift_raw > 1e3 * drift
+
+
+def test_residual_matches_manual(setup):
+    qbf, S, wl, e, net, W, torch = setup
+    isc, osc = 2.0, 0.05
+    cells = np.array([3, 17, 40])
+    r = net.residual(e, W, cells, isc, osc)
+    Ws = W[torch.from_numpy(cells).cuda()].contiguous()
+    Cex = S.exact_collision_phys(e, Ws)
+    Cn = net.forward(Ws.float().contiguous(), isc, osc).double()
+    want = float((Cn - Cex).abs().sum() / Cex.abs().sum())
+    print("residual", r, "manual", want)
+    assert abs(r - want) <= 1e-9 * want
+    with pytest.raises(qbf.QBFError):
+        net.residual(e, W, np.array([64]), isc, osc)
+
+
+def test_policy_exact_and_fallback_reproduce_exact_steps(setup):
+    qbf, S, wl, e, net, W, torch = setup
+    h, dt = 1.0 / 64, wl.cfl_dt(N, 64)
+    ref = W.clone()
+    e.set_moment_correction(qbf.CONS_ALL)
+    for _ in range(3):
+        ref = e.inhom_step_periodic(ref, wl.L_BOX, h, dt)
+    for kw in ({"exact_every": 1}, {"exact_every": 0, "tol": 0.0, "sample_frac": 0.05}):
+        st = S.SurrogateStepper(e, net, 2.0, 0.05, wl.L_BOX, h, dt, **kw)
+        X = W.clone()
+        for _ in range(3):
+            X = st.step(X)
+        print(kw, st.counts)
+        assert torch.equal(X, ref)
+    assert st.counts["fallback"] == 3
+
+
+def test_safeguards_reduce_error_of_a_rough_surrogate(setup):
+    """A briefly trained network: with the moment correction and an exact step every 5 steps
+    the error after 10 steps against the exact trajectory is smaller than with the bare
+    surrogate, and mass, momentum and energy are conserved to rounding."""
+    qbf, S, wl, e, net, W, torch = setup
+    from qbf.cnn import CNN
+    X, Y = S.training_set(e, seeds=[21, 22], times=[0.0, 0.02], cells_per_state=32, Nx=64, chunk=64)
+    isc, osc = S.scales(X, Y)
+    rough = CNN(max_batch=64, seed=2)
+    S.train(rough, X, Y, isc, osc, steps=150, batch=16, lr=2e-3)
+    h, dt = 1.0 / 64, wl.cfl_dt(N, 64)
+    e.set_moment_correction(qbf.CONS_ALL)
+    ref = W.clone()
+    for _ in range(10):
+        ref = e.inhom_step_periodic(ref, wl.L_BOX, h, dt)
+    res = {}
+    for name, kw in (("bare", dict(exact_every=0, check=False, mask=0)),
+                     ("safe", dict(exact_every=5, check=False, mask=qbf.CONS_ALL))):
+        st = S.SurrogateStepper(e, rough, isc, osc, wl.L_BOX, h, dt, **kw)
+        Z = W.clone()
+        for _ in range(10):
+            Z = st.step(Z)
+        z, r, w0 = Z.cpu().numpy(), ref.cpu().numpy(), W.cpu().numpy()
+        m0, mz = wl.moments(w0).sum(0), wl.moments(z).sum(0)
+        res[name] = (np.abs(z - r).sum() / np.abs(r - w0).sum(),
+                     max(abs(mz[k] - m0[k]) / abs(m0[k]) for k in (0, 6)))
+        print(name, "error vs exact change", res[name][0], "mass/energy drift", res[name][1], st.counts)
+    assert res["safe"][0] < res["bare"][0]
+    assert res["safe"][1] < 1e-12
