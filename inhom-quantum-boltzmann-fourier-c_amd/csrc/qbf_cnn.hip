@@ -253,15 +253,17 @@ __global__ void k_to_f32(const double* __restrict__ W, float* __restrict__ x, lo
 __global__ void k_half_f32(const double* __restrict__ W, const float* __restrict__ Cv, double a, float* __restrict__ x,
 	long long n)
 {
+#pragma clang fp contract(off)
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) x[i] = (float)__dadd_rn(W[i], __dmul_rn(a, (double)Cv[i]));
+	if (i < n) x[i] = (float)(W[i] + a * (double)Cv[i]);
 }
 // out = W + a C
 __global__ void k_update_f64(const double* __restrict__ W, const float* __restrict__ Cv, double a, double* __restrict__ out,
 	long long n)
 {
+#pragma clang fp contract(off)
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) out[i] = __dadd_rn(W[i], __dmul_rn(a, (double)Cv[i]));
+	if (i < n) out[i] = W[i] + a * (double)Cv[i];
 }
 
 // gather cells idx[j] of W [*][4][S][S] into out [nsel][4][S][S]
