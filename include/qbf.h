@@ -122,6 +122,10 @@ const char *qbf_cnn_last_error(void);
 int    qbf_cnn_create(int N, int maxBatch, qbf_cnn **out);
 void   qbf_cnn_destroy(qbf_cnn *c);
 int    qbf_cnn_set_stream(qbf_cnn *c, void *hip_stream);
+/* dilation per layer (dil[6], each in [1, 16]; NULL keeps the current ones) and per-cell
+ * normalisation (norm = 1: the network sees x / a_b and the channel iscale * a_b, a_b = max |x[b][0]|;
+ * its output is scaled by oscale * a_b^2 and the training loss is taken on the normalised output) */
+int    qbf_cnn_configure(qbf_cnn *c, const int *dil, int norm);
 size_t qbf_cnn_param_count(void);
 int    qbf_cnn_set_params(qbf_cnn *c, const float *host);
 int    qbf_cnn_get_params(qbf_cnn *c, float *host, int grads);

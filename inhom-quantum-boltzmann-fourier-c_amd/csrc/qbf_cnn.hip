@@ -12,6 +12,10 @@
 // MFMA operand layout (16 x 16 x 32): lane l holds A[l%16][8*(l/16) .. +7],
 // B[8*(l/16) .. +7][l%16], and the result D[4*(l/16) + r][l%16], r < 4.
 // The weight gradient (a reduction over all pixels) runs on the VALU in fp32.
+// Options (qbf_cnn_configure): a dilation d_l per layer (tap offsets (ky-1, kx-1) * d_l), and
+// per-cell normalisation: a_b = max |x[b][0]| (trace channel); the network sees x / a_b and the
+// constant channel iscale * a_b, its output is scaled by oscale * a_b^2, and the training loss
+// is taken on the normalised output.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -50,27 +54,49 @@ __device__ __forceinline__ float gelu_grad(float z)
 	return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
-// planar fp32 [B][4][S][S] -> NHWC bf16 [B][S][S][CIO] (channels >= 4 zero)
-__global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, int B, float scale)
+// a[b] = max |x[b][0][.]|  (one workgroup of 256 per cell)
+__global__ __launch_bounds__(256) void k_cell_amp(const float* __restrict__ x, float* __restrict__ amp)
+{
+	__shared__ float sh[4];
+	const float* xb = x + (size_t)blockIdx.x * CREAL * S * S;
+	float m = 0.f;
+	for (int p = threadIdx.x; p < S * S; p += 256) m = fmaxf(m, fabsf(xb[p]));
+	for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) amp[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+}
+
+// planar fp32 [B][4][S][S] -> NHWC bf16 [B][S][S][CIO]: channels 0..3 = scale * x (or
+// scale * x / a[b] with amp), channel 4 = scale * a[b] with amp, the rest zero
+__global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, int B, float scale,
+	const float* __restrict__ amp)
 {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	const long long n = (long long)B * S * S;
 	if (i >= n) return;
 	const int b = (int)(i / (S * S)), p = (int)(i % (S * S));
+	const float f = amp ? scale / fmaxf(amp[b], 1e-30f) : scale;
 	bf16* o = out + i * CIO;
-	for (int c = 0; c < CIO; c++) o[c] = (bf16)(c < CREAL ? scale * x[((size_t)b * CREAL + c) * (S * S) + p] : 0.0f);
+	for (int c = 0; c < CIO; c++) {
+		float v = 0.0f;
+		if (c < CREAL) v = x[((size_t)b * CREAL + c) * (S * S) + p] * f;
+		else if (c == CREAL && amp) v = scale * amp[b];
+		o[c] = (bf16)v;
+	}
 }
 
 // Implicit-GEMM 3x3 periodic convolution on the matrix cores.
 //   out[b][y][x][n] = sum_t sum_k in[b][y + DIR*(ky-1)][x + DIR*(kx-1)][k] * Wt[t][n][k]
 // DIR = +1: forward (Wt = W[t][cout][cin]); DIR = -1: backward data (Wt = W^T[t][cin][cout]).
 // MODE 0: out = bias + value, store pre-activation z (fp32) and GELU(z) (bf16 NHWC)
-// MODE 1: out = bias + value, store fp32 planar [B][4][S][S] (first 4 channels)
+// MODE 1: out = bias + value, store fp32 planar [B][4][S][S] (first 4 channels), times
+//         oscale (and a[b]^2 when amp is given)
 // MODE 2: backward data: g = value * gelu'(zprev) -> bf16 NHWC (gradient of the previous z)
 template <int KC, int NC, int DIR, int MODE>
 __global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ Wt,
 	const float* __restrict__ bias, float* __restrict__ zout, bf16* __restrict__ aout, float* __restrict__ yout,
-	const float* __restrict__ zprev, float oscale)
+	const float* __restrict__ zprev, float oscale, int dil, const float* __restrict__ amp)
 {
 	constexpr int NT = NC / 16;       // 16-wide output tiles
 	const int row = blockIdx.x;       // b * S + y
@@ -82,9 +108,9 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, 
 #pragma unroll
 	for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 	for (int ky = 0; ky < 3; ky++) {
-		const int ys = (y + DIR * (ky - 1) + S) & (S - 1);
+		const int ys = (y + DIR * dil * (ky - 1) + 2 * S) & (S - 1);
 		for (int kx = 0; kx < 3; kx++) {
-			const int xs = (x + DIR * (kx - 1) + S) & (S - 1);
+			const int xs = (x + DIR * dil * (kx - 1) + 2 * S) & (S - 1);
 			const bf16* src = in + (((size_t)b * S + ys) * S + xs) * KC;
 			const bf16* wt = Wt + (size_t)(ky * 3 + kx) * NC * KC;
 #pragma unroll
@@ -99,6 +125,8 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, 
 		}
 	}
 	// lane holds D[4*kq + r][i16]: pixel wave*16 + 4*kq + r, channel t*16 + i16
+	float osc = oscale;
+	if (MODE == 1 && amp) osc *= amp[b] * amp[b];
 #pragma unroll
 	for (int t = 0; t < NT; t++) {
 		const int n = t * 16 + i16;
@@ -112,7 +140,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, 
 				if (zout) zout[pix * NC + n] = z;
 				aout[pix * NC + n] = (bf16)gelu(z);
 			} else if (MODE == 1) {
-				if (n < CREAL) yout[((size_t)b * CREAL + n) * (S * S) + (size_t)y * S + xo] = oscale * (v + bias[n]);
+				if (n < CREAL) yout[((size_t)b * CREAL + n) * (S * S) + (size_t)y * S + xo] = osc * (v + bias[n]);
 			} else {
 				aout[pix * NC + n] = (bf16)(v * gelu_grad(zprev[pix * NC + n]));
 			}
@@ -121,16 +149,17 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, 
 }
 
 // Weight and bias gradients on the VALU:
-//   dW[t][co][ci] += sum_p dz[p][co] * x[p + off_t][ci],  db[co] += sum_p dz[p][co]
+//   dW[t][co][ci] += sum_p dz[p][co] * x[p + dil off_t][ci],  db[co] += sum_p dz[p][co]
 // One workgroup per (image, block of ROWS rows); thread (co, 16-channel ci block) keeps
-// 9 x 16 accumulators; the rows of x (with a one-pixel halo) and dz are staged in LDS.
+// 9 x 16 accumulators; the three rows of x that a row of dz meets and that row of dz are
+// staged in LDS (periodic in x).
 template <int CI, int CO>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dz,
-	float* __restrict__ dW, float* __restrict__ db, int rows)
+	float* __restrict__ dW, float* __restrict__ db, int rows, int dil)
 {
 	constexpr int NCB = CI / 16;                 // ci blocks of 16
 	constexpr int NTH = CO * NCB;                // active threads
-	__shared__ float xs[3][S + 2][CI + 1];
+	__shared__ float xs[3][S][CI + 1];
 	__shared__ float zs[S][CO + 1];
 	const int b = blockIdx.x / (S / rows);
 	const int y0 = (blockIdx.x % (S / rows)) * rows;
@@ -144,11 +173,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, 
 	float bacc = 0.f;
 	for (int y = y0; y < y0 + rows; y++) {
 		__syncthreads();
-		for (int e = tid; e < 3 * (S + 2) * CI; e += blockDim.x) {
-			const int r = e / ((S + 2) * CI), rem = e % ((S + 2) * CI);
+		for (int e = tid; e < 3 * S * CI; e += blockDim.x) {
+			const int r = e / (S * CI), rem = e % (S * CI);
 			const int xx = rem / CI, c = rem % CI;
-			const int ys = (y + r - 1 + S) & (S - 1), xsrc = (xx - 1 + S) & (S - 1);
-			xs[r][xx][c] = (float)x[(((size_t)b * S + ys) * S + xsrc) * CI + c];
+			const int ys = (y + dil * (r - 1) + 2 * S) & (S - 1);
+			xs[r][xx][c] = (float)x[(((size_t)b * S + ys) * S + xx) * CI + c];
 		}
 		for (int e = tid; e < S * CO; e += blockDim.x) {
 			const int xx = e / CO, c = e % CO;
@@ -177,23 +206,25 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, 
 	}
 }
 
-// loss = sum w (pred - target)^2 / n  (mean over elements), gradient -> bf16 NHWC (CIO channels)
+// loss = mean over elements of d^2, gradient with respect to the network output -> bf16 NHWC
+// (CIO channels).  d = pred - target, or (pred - target) / (oscale a[b]^2) with amp.
 __global__ void k_loss_grad(const float* __restrict__ pred, const float* __restrict__ target, bf16* __restrict__ g,
-	float* __restrict__ loss, int B, float inv_n, float oscale)
+	float* __restrict__ loss, int B, float inv_n, float oscale, const float* __restrict__ amp)
 {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	const long long n = (long long)B * S * S;
 	float part = 0.f;
 	if (i < n) {
 		const int b = (int)(i / (S * S)), p = (int)(i % (S * S));
+		const float sc = amp ? oscale * amp[b] * amp[b] : 1.0f;
 		bf16* o = g + i * CIO;
 		for (int c = 0; c < CIO; c++) {
 			float gv = 0.f;
 			if (c < CREAL) {
 				const size_t e = ((size_t)b * CREAL + c) * (S * S) + p;
-				const float d = pred[e] - target[e];
+				const float d = amp ? (pred[e] - target[e]) / sc : pred[e] - target[e];
 				part += d * d;
-				gv = 2.0f * d * inv_n * oscale;   // d loss / d (network output before oscale)
+				gv = amp ? 2.0f * d * inv_n : 2.0f * d * inv_n * oscale;   // d loss / d (network output)
 			}
 			o[c] = (bf16)gv;
 		}
@@ -335,6 +366,9 @@ struct qbf_cnn {
 	float* z[NL] = {};
 	bf16* g[NL] = {};     // gradient of z[l] (bf16 NHWC), g[NL-1] = gradient of the output
 	float* loss = nullptr;
+	int dil[NL] = {1, 1, 1, 1, 1, 1};
+	int norm = 0;                         // per-cell normalisation
+	float* amp = nullptr;                 // per-cell amplitude [maxB]
 	float *sx = nullptr, *sc = nullptr;   // surrogate step: network input / output, fp32 [maxB][4][S][S]
 	double* sw = nullptr;                 // surrogate inhom step: state after the first half transport
 	double* sw2 = nullptr;                // ... and after the collision step
@@ -393,6 +427,7 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		}
 	}
 	c->loss = c->alloc_f(1);
+	c->amp = c->alloc_f(maxBatch);
 	c->sx = c->alloc_f(px * CREAL);
 	c->sc = c->alloc_f(px * CREAL);
 	{
@@ -416,7 +451,7 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		c->swc = (double*)p3;
 		c->sres = (double*)p4;
 	}
-	if (!c->loss || !c->sx || !c->sc || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
+	if (!c->loss || !c->amp || !c->sx || !c->sc || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
 		delete c;
 		return cnn_err(-3, "qbf_cnn_create: out of device memory");
 	}
@@ -441,6 +476,19 @@ int qbf_cnn_set_stream(qbf_cnn* c, void* s)
 {
 	if (!c) return cnn_err(-1, "bad argument");
 	c->stream = s ? (hipStream_t)s : c->own;
+	return 0;
+}
+
+// dilation per layer (1 <= d <= 16; NULL keeps the current ones) and per-cell normalisation (0/1)
+int qbf_cnn_configure(qbf_cnn* c, const int* dil, int norm)
+{
+	if (!c || (norm != 0 && norm != 1)) return cnn_err(-1, "qbf_cnn_configure: bad argument");
+	if (dil)
+		for (int l = 0; l < NL; l++)
+			if (dil[l] < 1 || dil[l] > 16) return cnn_err(-1, "qbf_cnn_configure: dilation must be in [1, 16]");
+	if (dil)
+		for (int l = 0; l < NL; l++) c->dil[l] = dil[l];
+	c->norm = norm;
 	return 0;
 }
 
@@ -492,15 +540,18 @@ int qbf_cnn_get_params(qbf_cnn* c, float* host, int grads)
 static int run_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale, bool train)
 {
 	const long long npx = (long long)B * S * S;
-	hipLaunchKernelGGL(k_to_nhwc, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B, iscale);
+	const float* amp = c->norm ? c->amp : nullptr;
+	if (c->norm) hipLaunchKernelGGL(k_cell_amp, dim3(B), dim3(256), 0, c->stream, dIn, c->amp);
+	hipLaunchKernelGGL(k_to_nhwc, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B, iscale,
+		amp);
 	const dim3 grid(B * S), blk(256);
 	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
-		train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f);
+		train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f, c->dil[0], nullptr);
 	for (int l = 1; l < NL - 1; l++)
 		hipLaunchKernelGGL((k_conv_mfma<CH, CH, 1, 0>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
-			train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f);
+			train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f, c->dil[l], nullptr);
 	hipLaunchKernelGGL((k_conv_mfma<CH, CIO, 1, 1>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
-		c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale);
+		c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale, c->dil[NL - 1], amp);
 	HCHK(hipGetLastError());
 	return 0;
 }
@@ -527,21 +578,23 @@ int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B,
 	const long long npx = (long long)B * S * S;
 	const float inv_n = 1.0f / (float)(npx * CREAL);
 	hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dOut, dTarget,
-		c->g[NL - 1], c->loss, B, inv_n, oscale);
+		c->g[NL - 1], c->loss, B, inv_n, oscale, c->norm ? c->amp : nullptr);
 	const dim3 grid(B * S), blk(256);
 	const int rows = 16;
 	const dim3 wgrid(B * (S / rows));
 	// last layer: weight gradient, then data gradient into the last hidden pre-activation
 	hipLaunchKernelGGL((k_conv_wgrad<CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gw[NL - 1],
-		c->gb[NL - 1], rows);
+		c->gb[NL - 1], rows, c->dil[NL - 1]);
 	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, -1, 2>), grid, blk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
-		nullptr, c->g[NL - 2], nullptr, c->z[NL - 2], 1.0f);
+		nullptr, c->g[NL - 2], nullptr, c->z[NL - 2], 1.0f, c->dil[NL - 1], nullptr);
 	for (int l = NL - 2; l >= 1; l--) {
-		hipLaunchKernelGGL((k_conv_wgrad<CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows);
+		hipLaunchKernelGGL((k_conv_wgrad<CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows,
+			c->dil[l]);
 		hipLaunchKernelGGL((k_conv_mfma<CH, CH, -1, 2>), grid, blk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
-			c->g[l - 1], nullptr, c->z[l - 1], 1.0f);
+			c->g[l - 1], nullptr, c->z[l - 1], 1.0f, c->dil[l], nullptr);
 	}
-	hipLaunchKernelGGL((k_conv_wgrad<CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows);
+	hipLaunchKernelGGL((k_conv_wgrad<CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows,
+		c->dil[0]);
 	HCHK(hipGetLastError());
 	if (hostLoss) {
 		HCHK(hipMemcpyAsync(hostLoss, c->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -558,7 +611,7 @@ int qbf_cnn_adam(qbf_cnn* c, float lr, float b1, float b2, float eps)
 	const float bc1 = 1.0f - powf(b1, (float)c->step), bc2 = 1.0f - powf(b2, (float)c->step);
 	for (int l = 0; l < NL; l++) {
 		const int co = cout_of(l), ci = cin_of(l);
-		const int co_r = l == NL - 1 ? CREAL : CH, ci_r = l == 0 ? CREAL : CH;
+		const int co_r = l == NL - 1 ? CREAL : CH, ci_r = l == 0 ? (c->norm ? CREAL + 1 : CREAL) : CH;
 		const int n = (int)wsize(l);
 		hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->w[l], c->mw[l], c->vw[l], c->gw[l],
 			c->wb[l], c->wbt[l], co, ci, co_r, ci_r, n, lr, b1, b2, eps, bc1, bc2, 0);

@@ -29,6 +29,7 @@ def _lib():
             "qbf_cnn_destroy": (None, [vp]),
             "qbf_cnn_set_stream": (i, [vp, vp]),
             "qbf_cnn_param_count": (sz, []),
+            "qbf_cnn_configure": (i, [vp, C.POINTER(i), i]),
             "qbf_cnn_set_params": (i, [vp, fp]),
             "qbf_cnn_get_params": (i, [vp, fp, i]),
             "qbf_cnn_forward": (i, [vp, vp, i, f, vp, f]),
@@ -61,15 +62,16 @@ def param_count() -> int:
     return int(_lib().qbf_cnn_param_count())
 
 
-def init_params(seed: int = 0) -> np.ndarray:
-    """He-normal weights on the used channels, zero biases and zero padded entries; the last
-    layer is scaled by 0.1 so that an untrained network predicts small values."""
+def init_params(seed: int = 0, norm: bool = False) -> np.ndarray:
+    """He-normal weights on the used channels (4 input channels, 5 with per-cell normalisation),
+    zero biases and zero padded entries; the last layer is scaled by 0.1 so that an untrained
+    network predicts small values."""
     rng = np.random.default_rng(seed)
     out = []
     for l in range(NL):
         ci = CIO if l == 0 else CH
         co = CIO if l == NL - 1 else CH
-        ci_r = CREAL if l == 0 else CH
+        ci_r = (CREAL + 1 if norm else CREAL) if l == 0 else CH
         co_r = CREAL if l == NL - 1 else CH
         W = np.zeros((9, co, ci), np.float32)
         W[:, :co_r, :ci_r] = rng.standard_normal((9, co_r, ci_r)) * np.sqrt(2.0 / (9 * ci_r))
@@ -82,7 +84,8 @@ def init_params(seed: int = 0) -> np.ndarray:
 class CNN:
     """Six-layer periodic 3x3 CNN (bf16 weights/activations, fp32 accumulation) on the GPU."""
 
-    def __init__(self, max_batch: int, params: np.ndarray | None = None, seed: int = 0):
+    def __init__(self, max_batch: int, params: np.ndarray | None = None, seed: int = 0, dil=None,
+                 norm: bool = False):
         import torch
         if not torch.cuda.is_available():
             raise QBFError("no HIP device available")
@@ -91,7 +94,11 @@ class CNN:
         _check(_lib().qbf_cnn_create(S, max_batch, C.byref(h)))
         self._h = h
         self.max_batch = max_batch
-        self.set_params(init_params(seed) if params is None else params)
+        self.dil = tuple(dil) if dil is not None else (1,) * NL
+        self.norm = bool(norm)
+        d = (C.c_int * NL)(*self.dil)
+        _check(_lib().qbf_cnn_configure(self._h, d, 1 if norm else 0))
+        self.set_params(init_params(seed, norm) if params is None else params)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -41,15 +41,24 @@ def make_engine(N=N_SURR, max_cells=512, J=wl.J_ANG, M=wl.M_GL, precision=FP32):
     return e
 
 
-def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=256, rng_seed=0):
+DILATION = (1, 2, 4, 8, 4, 1)      # receptive field 43 x 43 pixels: the collision ball has ~40
+
+
+def make_net(max_batch, seed=0):
+    """The surrogate network: dilated, with per-cell normalisation."""
+    return CNN(max_batch=max_batch, seed=seed, dil=DILATION, norm=True)
+
+
+def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, rng_seed=0):
     """(X, Y): float32 cuda tensors [n, 4, 64, 64] of states and their exact collision operator."""
     import torch
     rng = np.random.default_rng(rng_seed)
+    chunk = chunk or engine.max_cells
     X, Y = [], []
     for s in seeds:
         for t in times:
             cells = np.sort(rng.choice(Nx, cells_per_state, replace=False))
-            W = wl.periodic(N_SURR, Nx, seed=int(s), t=float(t))[cells]
+            W = wl.periodic(N_SURR, Nx, seed=int(s), t=float(t), cells=cells)
             X.append(torch.from_numpy(W).cuda())
     Xd = torch.cat(X)
     for i in range(0, Xd.shape[0], chunk):
@@ -58,13 +67,19 @@ def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=256, rn
     return Xd.float().contiguous(), Yd.float().contiguous()
 
 
-def scales(X, Y):
-    """iscale maps the states to O(1), oscale the collision operator."""
-    return 1.0 / float(X.abs().max()), float(Y.abs().max())
+def scales(X, Y, norm=True):
+    """iscale maps the states to O(1); oscale the collision operator (per unit a^2 with the
+    per-cell normalisation, a = max |W0| of the cell)."""
+    if not norm:
+        return 1.0 / float(X.abs().max()), float(Y.abs().max())
+    a = X[:, 0].abs().amax(dim=(1, 2))
+    return 1.0 / float(a.max()), float((Y.abs().amax(dim=(1, 2, 3)) / (a * a)).max())
 
 
 def rel_l1(net, X, Y, iscale, oscale, batch=256):
+    """sum |net(X) - Y| / sum |Y|"""
     num = den = 0.0
+    batch = min(batch, net.max_batch)
     for i in range(0, X.shape[0], batch):
         P = net.forward(X[i:i + batch].contiguous(), iscale, oscale)
         num += float((P - Y[i:i + batch]).abs().sum())
@@ -77,14 +92,16 @@ def train(net, X, Y, iscale, oscale, steps=3000, batch=32, lr=2e-3, lr_min=2e-5,
     target Y / oscale.  Returns the loss history."""
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
-    Tn = (Y / oscale).contiguous()
+    # without per-cell normalisation the target is normalised here; with it, by the kernels
+    Tn = Y.contiguous() if net.norm else (Y / oscale).contiguous()
+    osc = oscale if net.norm else 1.0
     n = X.shape[0]
     hist = []
     for it in range(steps):
         idx = torch.randint(0, n, (batch,), generator=g).cuda()
         xb = X.index_select(0, idx).contiguous()
         tb = Tn.index_select(0, idx).contiguous()
-        loss, _ = net.gradients(xb, tb, iscale, 1.0)
+        loss, _ = net.gradients(xb, tb, iscale, osc)
         a = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / max(1, steps - 1)))
         net.adam(a)
         hist.append(loss)

@@ -44,14 +44,15 @@ def homogeneous(N: int = 32, seed: int = 0, L: float = L_BOX) -> np.ndarray:
     return W
 
 
-def periodic(N: int, Nx: int, seed: int = 0, kmax: int = 4, L: float = L_BOX, t: float = 0.0) -> np.ndarray:
+def periodic(N: int, Nx: int, seed: int = 0, kmax: int = 4, L: float = L_BOX, t: float = 0.0,
+             cells=None) -> np.ndarray:
     """Periodic inhomogeneous state on cells x_i = (i + 1/2)/Nx of [0, 1).  w0 is a Maxwellian in v
     whose density and temperature are 1 + 0.3 sin(2 pi x) + sum_{k<=kmax} 0.05 sin(2 pi k x + phase_k)
     (seeded phases), scaled so that max w0 = 0.5 at t = 0; the Pauli direction rotates about a
     seeded axis by an integer number of turns over the period.  With t > 0 the state is the free
     streaming of that initial state, W(x, v, t) = W(x - v_x t, v, 0) (transport without
     collisions), which is how the surrogate's training states leave local equilibrium.
-    Returns [Nx, 4, N, N]."""
+    Returns [Nx, 4, N, N], or only the listed cells."""
     rng = np.random.default_rng(seed)
     ph_rho = rng.uniform(0, 2 * np.pi, kmax)
     ph_T = rng.uniform(0, 2 * np.pi, kmax)
@@ -70,10 +71,12 @@ def periodic(N: int, Nx: int, seed: int = 0, kmax: int = 4, L: float = L_BOX, t:
     x = (np.arange(Nx) + 0.5) / Nx
     w0_init, _ = profiles(x[:, None, None])
     scale = 0.5 / w0_init.max()
+    if cells is not None:
+        x = x[np.asarray(cells)]
     xs = x[:, None, None] - vx[None] * t          # foot of the characteristic, per cell and velocity
     w0, n = profiles(xs)
     w0 = w0 * scale
-    W = np.empty((Nx, 4, N, N))
+    W = np.empty((len(x), 4, N, N))
     W[:, 0] = w0
     for i in range(3):
         W[:, i + 1] = 0.8 * w0 * n[i]

@@ -11,9 +11,12 @@ from oracle import cnn_ref as R
 pytestmark = pytest.mark.gpu
 
 
-def _setup(seed, B=3, S=64):
+CONFIGS = [dict(dil=(1,) * 6, norm=False), dict(dil=(1, 2, 4, 8, 4, 1), norm=True)]
+
+
+def _setup(seed, B=3, S=64, norm=False):
     rng = np.random.default_rng(seed)
-    p = R.init_params(seed)
+    p = R.init_params(seed, cin0=5 if norm else 4)
     o = 0
     for l in range(R.NL):
         o += 9 * R.cout(l) * R.cin(l)
@@ -24,6 +27,7 @@ def _setup(seed, B=3, S=64):
     o5 = o - R.cout(R.NL - 1) - 9 * R.cout(R.NL - 1) * R.cin(R.NL - 1)
     p[o5:o5 + 9 * R.cout(R.NL - 1) * R.cin(R.NL - 1)] *= 10
     x = (0.5 * rng.standard_normal((B, 4, S, S))).astype(np.float32)
+    x[1:2] *= 0.3                       # cells of different amplitude
     t = (0.3 * rng.standard_normal((B, 4, S, S))).astype(np.float32)
     return p.astype(np.float32), x, t
 
@@ -32,14 +36,15 @@ def _rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def test_forward_matches_oracle():
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_forward_matches_oracle(cfg):
     import torch
     from qbf.cnn import CNN
-    p, x, _ = _setup(0)
-    net = CNN(max_batch=4, params=p)
+    p, x, _ = _setup(0, norm=cfg["norm"])
+    net = CNN(max_batch=4, params=p, **cfg)
     out = net.forward(torch.from_numpy(x).cuda(), iscale=1.5, oscale=0.8).cpu().numpy()
-    ref, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=True)
-    ref64, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=False)
+    ref, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=True, **cfg)
+    ref64, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=False, **cfg)
     e = _rel(out, ref)
     print("forward rel L2 vs bf16 oracle", e, " vs fp64 network", _rel(out, ref64))
     assert e < 1e-2
@@ -61,14 +66,15 @@ def test_forward_is_translation_equivariant_and_batch_independent():
     np.testing.assert_allclose(y1[0], y[1], rtol=0, atol=tol)
 
 
-def test_gradients_match_oracle():
+@pytest.mark.parametrize("cfg", CONFIGS)
+def test_gradients_match_oracle(cfg):
     import torch
     from qbf.cnn import CNN
-    p, x, t = _setup(2, B=2)
-    net = CNN(max_batch=2, params=p)
+    p, x, t = _setup(2, B=2, norm=cfg["norm"])
+    net = CNN(max_batch=2, params=p, **cfg)
     loss, out = net.gradients(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), iscale=1.2, oscale=0.9)
     g = net.get_params(grads=True)
-    lref, gref, oref = R.gradients(x, t, p, iscale=1.2, oscale=0.9, bf16_round=True)
+    lref, gref, oref = R.gradients(x, t, p, iscale=1.2, oscale=0.9, bf16_round=True, **cfg)
     assert abs(loss - lref) <= 5e-3 * lref, (loss, lref)
     assert _rel(out.cpu().numpy(), oref) < 1e-2
     o = 0

@@ -90,10 +90,9 @@ def test_safeguards_reduce_error_of_a_rough_surrogate(setup):
     the error after 10 steps against the exact trajectory is smaller than with the bare
     surrogate, and mass, momentum and energy are conserved to rounding."""
     qbf, S, wl, e, net, W, torch = setup
-    from qbf.cnn import CNN
     X, Y = S.training_set(e, seeds=[21, 22], times=[0.0, 0.02], cells_per_state=32, Nx=64, chunk=64)
     isc, osc = S.scales(X, Y)
-    rough = CNN(max_batch=64, seed=2)
+    rough = S.make_net(64, seed=2)
     S.train(rough, X, Y, isc, osc, steps=150, batch=16, lr=2e-3)
     h, dt = 1.0 / 64, wl.cfl_dt(N, 64)
     e.set_moment_correction(qbf.CONS_ALL)

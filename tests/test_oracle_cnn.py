@@ -43,13 +43,28 @@ def test_conv_delta_response():
         np.testing.assert_array_equal(y[0, 2 - (ky - 1), 3 - (kx - 1)], W[t][:, 1])
 
 
-def test_data_and_weight_gradients_are_adjoints():
+@pytest.mark.parametrize("d", [1, 3])
+def test_data_and_weight_gradients_are_adjoints(d):
     rng = np.random.default_rng(1)
     x, W = _rand_conv(rng)
-    g = rng.standard_normal(R.conv(x, W).shape)
-    lhs = (R.conv(x, W) * g).sum()
-    assert abs(lhs - (x * R.conv_data_grad(g, W)).sum()) < 1e-9 * abs(lhs) + 1e-9
-    assert abs(lhs - (W * R.conv_weight_grad(g, x)).sum()) < 1e-9 * abs(lhs) + 1e-9
+    g = rng.standard_normal(R.conv(x, W, d).shape)
+    lhs = (R.conv(x, W, d) * g).sum()
+    assert abs(lhs - (x * R.conv_data_grad(g, W, d)).sum()) < 1e-9 * abs(lhs) + 1e-9
+    assert abs(lhs - (W * R.conv_weight_grad(g, x, d)).sum()) < 1e-9 * abs(lhs) + 1e-9
+
+
+def test_dilated_conv_matches_direct_sum():
+    """Direct loop over pixels and taps with offsets d (ky-1, kx-1), periodic."""
+    rng = np.random.default_rng(2)
+    x, W = _rand_conv(rng, B=1, S=6, ci=2, co=2)
+    d = 2
+    y = np.zeros((1, 6, 6, 2))
+    for py in range(6):
+        for px in range(6):
+            for t in range(9):
+                ky, kx = divmod(t, 3)
+                y[0, py, px] += W[t] @ x[0, (py + d * (ky - 1)) % 6, (px + d * (kx - 1)) % 6]
+    np.testing.assert_allclose(R.conv(x, W, d), y, atol=1e-12)
 
 
 def test_gelu_close_to_erf_gelu_and_its_derivative():
@@ -76,13 +91,14 @@ def test_param_layout():
     assert np.all(b0 == 0)
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_backward_matches_finite_differences(seed):
+@pytest.mark.parametrize("seed,dil,norm", [(0, (1,) * 6, False), (1, (1,) * 6, False),
+                                           (2, (1, 2, 4, 8, 4, 1), True)])
+def test_backward_matches_finite_differences(seed, dil, norm):
     """float64 mode (no rounding): the analytic gradient equals central differences of the loss
-    for weights and biases of every layer."""
+    for weights and biases of every layer (plain, and dilated with per-cell normalisation)."""
     rng = np.random.default_rng(seed)
-    S, B = 8, 2
-    p = R.init_params(seed).astype(np.float64)
+    S, B = (8, 2) if not norm else (16, 2)
+    p = R.init_params(seed, cin0=5 if norm else 4).astype(np.float64)
     # nonzero biases and a larger last layer so that every term contributes
     layers_off = []
     o = 0
@@ -93,20 +109,23 @@ def test_backward_matches_finite_differences(seed):
         p[o:o + (4 if l == R.NL - 1 else 64)] = 0.1 * rng.standard_normal(4 if l == R.NL - 1 else 64)
         o += R.cout(l)
     x = rng.standard_normal((B, 4, S, S)) * 0.5
+    if norm:
+        x[1] *= 0.3                     # cells of different amplitude
     target = rng.standard_normal((B, 4, S, S)) * 0.1
 
     # the oracle casts parameters through float32: perturb in float32-representable steps
     p = p.astype(np.float32).astype(np.float64)
-    loss, g, _ = R.gradients(x, target, p, iscale=1.3, oscale=0.7, bf16_round=False)
+    kw = dict(iscale=1.3, oscale=0.7, bf16_round=False, dil=dil, norm=norm)
+    loss, g, _ = R.gradients(x, target, p, **kw)
 
     def L(q):
-        return R.gradients(x, target, q, iscale=1.3, oscale=0.7, bf16_round=False)[0]
+        return R.gradients(x, target, q, **kw)[0]
 
     idx = []
     for l in range(R.NL):
         o, nw = layers_off[l]
         W = np.zeros((9, R.cout(l), R.cin(l)), bool)
-        W[:, :(4 if l == R.NL - 1 else 64), :(4 if l == 0 else 64)] = True
+        W[:, :(4 if l == R.NL - 1 else 64), :((5 if norm else 4) if l == 0 else 64)] = True
         live = np.flatnonzero(W.ravel())
         idx += list(o + rng.choice(live, 3, replace=False))
         idx.append(o + nw + int(rng.integers(0, 4 if l == R.NL - 1 else 64)))
