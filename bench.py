@@ -1,15 +1,23 @@
 """Benchmark: collision evals/s of the Strang-split collision + transport step at N=64, Nx=512.
 
 One step = half transport, collision midpoint step (2 Cc+Cd evaluations per cell), half
-transport, over all Nx=512 cells on one GPU: 1024 collision evals per step.  The fast path
-(FP32 convolution kernels, FP64 state and time update) is timed for --steps steps after
---warmup steps; the FP64 plain path is timed on the same input for comparison, and the
-error of the fast path against it is reported.  Inputs are synthetic (seeded generator of
-qbf.workloads.periodic), resident in HBM before the timed region starts.
+transport, over all Nx=512 cells on one GPU: 1024 collision evals per step.  Paths:
 
-The CPU baseline (the oracle, a C restatement of the reference algorithm) is timed first,
-before the GPU is touched: one full Cc+Cd evaluation at the same N, J, M per process, one
-process per core.
+  surrogate  the fast path: CNN surrogate of Cc+Cd (BF16 matrix cores) with FP64 conservation
+             correction, an exact step every 10 steps and a residual check of 2% of the cells
+             before every other step (fallback to the exact step above the tolerance).  Its
+             network is trained on the GPU before the timed region, on states of other seeds.
+  mixed      FP32 convolution kernels, FP64 state, midpoint update and conservation correction.
+  fp64       FP64 plain: the reference's arithmetic.
+
+--warmup W steps run from the initial state and are discarded; then exactly --steps K steps of
+the fast path are timed from the initial state.  After K steps (200 by default) the density and
+energy per cell are compared with the FP64 plain trajectory in tests/golden/periodic64_fp64_ref.npz
+(frames every 20 steps).  Inputs are synthetic (qbf.workloads.periodic, seed 3), resident in HBM
+before the timed region.
+
+The CPU baseline (the oracle, a C restatement of the reference algorithm) is timed first, before
+the GPU is touched: one full Cc+Cd evaluation at the same N, J, M per process, one process per core.
 """
 from __future__ import annotations
 
@@ -27,7 +35,14 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 METRIC = "collision evals/s at N=64,Nx=512: FP32/surrogate vs FP64 plain, ≤1e-3 err"
-PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}
+PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "bf16": 2.5e15}
+REF_FILE = os.path.join(ROOT, "tests", "golden", "periodic64_fp64_ref.npz")
+
+# HBM traffic of one k_i3acc<float, 64, 1024> launch of this workload (FETCH_SIZE + WRITE_SIZE,
+# rocprofv3 --pmc on `bench.py --no-cpu-baseline --paths mixed --mixed-steps 1 --warmup 0
+# --copy-calibration`, each counter scaled by its reading for a 256 MiB device copy).
+I3_TRAFFIC_BYTES = None
+
 
 def cpu_baseline(N, J, M, cores, seed):
     """One full reference-formulation Cc+Cd evaluation (oracle) per process, `cores` processes."""
@@ -62,15 +77,23 @@ def cpu_baseline(N, J, M, cores, seed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps of the fast path")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--N", type=int, default=64)
     ap.add_argument("--Nx", type=int, default=512)
     ap.add_argument("--J", type=int, default=32)
     ap.add_argument("--M", type=int, default=16)
-    ap.add_argument("--fp64-steps", type=int, default=2, help="FP64 plain steps (first one compared)")
+    ap.add_argument("--paths", default="surrogate,mixed,fp64")
+    ap.add_argument("--mixed-steps", type=int, default=3, help="timed steps of the FP32 exact path")
+    ap.add_argument("--fp64-steps", type=int, default=1, help="timed FP64 plain steps")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--exact-every", type=int, default=10)
+    ap.add_argument("--sample-frac", type=float, default=0.02)
+    ap.add_argument("--tol", type=float, default=1e-3,
+                    help="fallback above this relative state error projected to the next exact step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true", help="run a 256 MiB device copy first")
     ap.add_argument("--seed", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -80,6 +103,7 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py is a single-process, single-GPU benchmark")
         sys.exit(2)
+    paths = set(args.paths.split(","))
 
     N, Nx, J, M = args.N, args.Nx, args.J, args.M
     cpu = None
@@ -94,103 +118,210 @@ def main():
     from qbf import workloads as wl
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     W0 = torch.from_numpy(wl.periodic(N, Nx, seed=args.seed)).to(dev)
+    w0 = W0.cpu().numpy()
     h = 1.0 / Nx
     dt = wl.cfl_dt(N, Nx)
     tabs = qbf.quadrature(N, J, wl.L_BOX, wl.R_BALL, M)
+    evals_per_step = 2 * Nx
+    P = 2 * N
+    flop_per_k = 2 * 5 * P * P * np.log2(P * P)     # IFFT2 + FFT2 of a P x P complex grid
+    ref = None
+    if os.path.exists(REF_FILE):
+        z = np.load(REF_FILE)
+        if (int(z["N"]), int(z["Nx"]), int(z["J"]), int(z["M"]), int(z["seed"])) == (N, Nx, J, M, args.seed):
+            ref = z
 
     def make(prec):
         e = qbf.Engine(N, prec, max_cells=Nx)
         e.set_quadrature(*tabs)
         return e
 
-    # ---------------------------------------------------------------- fast path (FP32 kernels)
+    def run_steps(step, W, k, keep_at=()):
+        """k steps from W with two alternating buffers; returns the final state and copies of the
+        states after the steps listed in keep_at."""
+        bufs = [W.clone(), torch.empty_like(W)]
+        cur, kept = 0, {}
+        for i in range(1, k + 1):
+            step(bufs[cur], bufs[1 - cur])
+            cur = 1 - cur
+            if i in keep_at:
+                kept[i] = bufs[cur].clone()
+        return bufs[cur], kept
+
+    def moment_errors(W, step_no):
+        """density / energy relative errors per cell against the FP64 plain trajectory."""
+        if ref is None or step_no % int(ref["every"]) or step_no // int(ref["every"]) >= ref["moments"].shape[0]:
+            return None
+        mr = ref["moments"][step_no // int(ref["every"])]
+        m = wl.moments(W.cpu().numpy())
+        return {"step": step_no,
+                "density_max_rel": float(np.abs(m[:, 0] - mr[:, 0]).max() / np.abs(mr[:, 0]).max()),
+                "energy_max_rel": float(np.abs(m[:, 6] - mr[:, 6]).max() / np.abs(mr[:, 6]).max()),
+                "density_l1_rel": float(np.abs(m[:, 0] - mr[:, 0]).sum() / np.abs(mr[:, 0]).sum()),
+                "energy_l1_rel": float(np.abs(m[:, 6] - mr[:, 6]).sum() / np.abs(mr[:, 6]).sum())}
+
+    out = {}
+    check_step = args.steps if ref is not None and args.steps % int(ref["every"]) == 0 else None
+
+    # ---------------------------------------------------------------- FP32 exact path
     e32 = make(qbf.FP32)
-    e32.set_moment_correction(qbf.CONS_ALL)     # FP64 conservation correction of every collision step
-    # warmup: the first step from W0 is kept for the error comparison
-    W32_1 = e32.inhom_step_periodic(W0, wl.L_BOX, h, dt)
-    bufs = [W32_1.clone(), torch.empty_like(W0)]
-    cur = 0
-    for i in range(max(0, args.warmup - 1)):
-        e32.inhom_step_periodic(bufs[cur], wl.L_BOX, h, dt, out=bufs[1 - cur])
-        cur = 1 - cur
-    torch.cuda.synchronize()
-    e32.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        e32.inhom_step_periodic(bufs[cur], wl.L_BOX, h, dt, out=bufs[1 - cur])
-        cur = 1 - cur
-    torch.cuda.synchronize()
-    t32 = time.perf_counter() - t0
-    kms, nlaunch, kapps = e32.profile_read()
-    e32.profile(False)
-    evals = 2 * Nx * args.steps
-    value = evals / t32
+    e32.set_moment_correction(qbf.CONS_ALL)
+    if "mixed" in paths:
+        step32 = lambda a, b: e32.inhom_step_periodic(a, wl.L_BOX, h, dt, out=b)  # noqa: E731
+        W32_1, _ = run_steps(step32, W0, 1)
+        torch.cuda.synchronize()
+        e32.profile(True)
+        t0 = time.perf_counter()
+        run_steps(step32, W32_1, args.mixed_steps)
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - t0
+        kms, nl, kapps = e32.profile_read()
+        e32.profile(False)
+        out["mixed"] = {"value": evals_per_step * args.mixed_steps / tm, "ms_per_step": 1e3 * tm / args.mixed_steps,
+                        "steps": args.mixed_steps, "i3": (kms, nl, kapps, tm), "W1": W32_1}
 
     # ---------------------------------------------------------------- FP64 plain path
-    e64 = make(qbf.FP64)
-    W64_1 = e64.inhom_step_periodic(W0, wl.L_BOX, h, dt)
-    torch.cuda.synchronize()
-    b64 = [W64_1.clone(), torch.empty_like(W0)]
-    n64 = max(1, args.fp64_steps - 1)
-    t0 = time.perf_counter()
-    for i in range(n64):
-        e64.inhom_step_periodic(b64[i % 2], wl.L_BOX, h, dt, out=b64[(i + 1) % 2])
-    torch.cuda.synchronize()
-    t64 = time.perf_counter() - t0
-    value64 = 2 * Nx * n64 / t64
+    if "fp64" in paths:
+        e64 = make(qbf.FP64)
+        step64 = lambda a, b: e64.inhom_step_periodic(a, wl.L_BOX, h, dt, out=b)  # noqa: E731
+        W64_1, _ = run_steps(step64, W0, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_steps(step64, W64_1, args.fp64_steps)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        out["fp64"] = {"value": evals_per_step * args.fp64_steps / t64, "ms_per_step": 1e3 * t64 / args.fp64_steps,
+                       "steps": args.fp64_steps, "W1": W64_1}
+        del e64
 
-    a = W32_1.cpu().numpy()
-    b = W64_1.cpu().numpy()
-    m0 = wl.moments(W0.cpu().numpy()).sum(0)
-    ma = wl.moments(a).sum(0)
-    mb = wl.moments(b).sum(0)
-    def drift(m):
-        return {"mass": float(abs(m[0] - m0[0]) / abs(m0[0])),
-                "momentum": float(np.hypot(m[4] - m0[4], m[5] - m0[5]) / abs(m0[0])),
-                "energy": float(abs(m[6] - m0[6]) / abs(m0[6]))}
+    # ---------------------------------------------------------------- surrogate path
+    if "surrogate" in paths:
+        from qbf import surrogate as S
+        t0 = time.perf_counter()
+        X, Y = S.training_set(e32, seeds=range(100, 116), times=np.linspace(0.0, 0.06, 7), cells_per_state=64, Nx=Nx)
+        Xv, Yv = S.training_set(e32, seeds=[args.seed], times=[0.0, 0.03], cells_per_state=32, Nx=Nx, rng_seed=1)
+        isc, osc = S.scales(X, Y)
+        torch.cuda.synchronize()
+        t_data = time.perf_counter() - t0
+        net = S.make_net(Nx, seed=0)
+        t0 = time.perf_counter()
+        hist = S.train(net, X, Y, isc, osc, steps=args.train_steps)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        val = S.rel_l1(net, Xv, Yv, isc, osc)
+        del X, Y
+        # CNN forward timing on the full batch (the matrix-core kernels)
+        xb = W0.float().contiguous()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        net.forward(xb, isc, osc)
+        ev0.record()
+        for _ in range(5):
+            net.forward(xb, isc, osc)
+        ev1.record()
+        torch.cuda.synchronize()
+        cnn_ms = ev0.elapsed_time(ev1) / 5
 
-    err = {
-        "step1_state_l1rel": float(np.abs(a - b).sum() / np.abs(b).sum()),
-        "step1_increment_l1rel": float(np.abs(a - b).sum() / np.abs(b - W0.cpu().numpy()).sum()),
-        "density_rel": float(abs(ma[0] - mb[0]) / abs(mb[0])),
-        "energy_rel": float(abs(ma[6] - mb[6]) / abs(mb[6])),
-        "step1_drift_fast_path": drift(ma),
-        "step1_drift_fp64_plain": drift(mb),
-    }
+        def make_stepper():
+            return S.SurrogateStepper(e32, net, isc, osc, wl.L_BOX, h, dt, exact_every=args.exact_every,
+                                      sample_frac=args.sample_frac, tol=args.tol, mask=qbf.CONS_ALL, seed=args.seed)
+        warm = make_stepper()
+        run_steps(lambda a, b: warm.step(a, out=b), W0, args.warmup)
+        st = make_stepper()
+        torch.cuda.synchronize()
+        e32.profile(True)
+        t0 = time.perf_counter()
+        Ws, kept = run_steps(lambda a, b: st.step(a, out=b), W0, args.steps,
+                             keep_at=(check_step,) if check_step else ())
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        kms, nl, kapps = e32.profile_read()
+        e32.profile(False)
+        err = moment_errors(kept[check_step], check_step) if check_step else None
+        out["surrogate"] = {
+            "value": evals_per_step * args.steps / ts, "ms_per_step": 1e3 * ts / args.steps, "steps": args.steps,
+            "i3": (kms, nl, kapps, ts), "error_vs_fp64": err, "counts": dict(st.counts),
+            "residual_median": float(np.median(st.residuals)) if st.residuals else None,
+            "residual_max": float(np.max(st.residuals)) if st.residuals else None,
+            "operator_residual_median": float(np.median(st.op_residuals)) if st.op_residuals else None,
+            "training": {"samples": int(16 * 7 * 64), "steps": args.train_steps, "data_s": round(t_data, 2),
+                         "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
+                         "val_rel_l1_seed3": round(val, 5)},
+            "cnn_forward_ms_512_cells": round(cnn_ms, 3)}
 
-    P = 2 * N
-    flop_per_k = 2 * 5 * P * P * np.log2(P * P)     # IFFT2 + FFT2 of a P x P complex grid
+    # ---------------------------------------------------------------- report
+    fast = None
+    s = out.get("surrogate")
+    if s and s["error_vs_fp64"] and max(s["error_vs_fp64"]["density_max_rel"], s["error_vs_fp64"]["energy_max_rel"]) <= 1e-3:
+        fast = "surrogate"
+    elif "mixed" in out:
+        fast = "mixed"
+    elif s:
+        fast = "surrogate"
+    if fast is None:
+        print(json.dumps({"metric": METRIC, "value": None, "error": "no fast path run"}))
+        return
+    F = out[fast]
+    kms, nl, kapps, twall = F["i3"]
     achieved = kapps * flop_per_k / (kms / 1e3) if kms > 0 else 0.0
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(F["value"], 3),
         "unit": "collision evals/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": F["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t32 / args.steps, 3),
+        "ms_per_step": round(F["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "bf16+fp32" if fast == "surrogate" else "fp32",
         "data": "synthetic",
-        "config": {"workload": f"Strang collision+transport step, N={N}, Nx={Nx}, J={J}, M={M}, "
-                               f"L={wl.L_BOX}, R={wl.R_BALL}, dt={dt:.4g} (CFL 0.5); FP32 convolution kernels, "
-                               "FP64 state, midpoint update and conservation correction",
-                   "N": N, "Nx": Nx, "J": J, "M": M, "evals_per_step": 2 * Nx},
-        "roofline": {"bound": "mfma", "kernel": "k_i3acc (I3 K-operator: IFFT2 -> psi2 -> FFT2 per term)",
+        "config": {"workload": f"Strang collision+transport step of the periodic problem, N={N}, Nx={Nx}, J={J}, "
+                               f"M={M}, L={wl.L_BOX}, R={wl.R_BALL}, dt={dt:.4g} (CFL 0.5), seed {args.seed}; "
+                               + ("CNN surrogate collision step with FP64 conservation correction, exact FP32 step "
+                                  f"every {args.exact_every}, {100 * args.sample_frac:g}% residual check, fallback "
+                                  f"above {args.tol:g}" if fast == "surrogate" else
+                                  "FP32 convolution kernels, FP64 state, midpoint update and conservation correction"),
+                   "path": fast, "N": N, "Nx": Nx, "J": J, "M": M, "evals_per_step": evals_per_step},
+        "roofline": {"bound": "mfma", "kernel": "k_i3acc<float,64,1024> (I3 K-operator: IFFT2 -> psi2 -> FFT2)",
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": None,
-                     "launches": nlaunch, "avg_launch_ms": round(kms / max(1, nlaunch), 4),
-                     "kernel_share_of_step": round(kms / 1e3 / t32, 4)},
+                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": I3_TRAFFIC_BYTES,
+                     "launches": nl, "avg_launch_ms": round(kms / max(1, nl), 4),
+                     "kernel_share_of_timed": round(kms / 1e3 / twall, 4)},
         "cpu_baseline": cpu,
-        "fp64_plain": {"value": round(value64, 3), "unit": "collision evals/s",
-                       "ms_per_step": round(1e3 * t64 / n64, 3), "steps": n64},
-        "speedup_vs_fp64": round(value / value64, 3),
-        "error_vs_fp64": err,
     }
+    if "fp64" in out:
+        line["fp64_plain"] = {"value": round(out["fp64"]["value"], 3), "unit": "collision evals/s",
+                              "ms_per_step": round(out["fp64"]["ms_per_step"], 3), "steps": out["fp64"]["steps"]}
+        line["speedup_vs_fp64"] = round(F["value"] / out["fp64"]["value"], 3)
+    if "mixed" in out:
+        mx = out["mixed"]
+        line["mixed_fp32"] = {"value": round(mx["value"], 3), "ms_per_step": round(mx["ms_per_step"], 3),
+                              "steps": mx["steps"]}
+        if "fp64" in out:
+            a, b = mx["W1"].cpu().numpy(), out["fp64"]["W1"].cpu().numpy()
+            ma, mb = wl.moments(a), wl.moments(b)
+            line["mixed_fp32"]["step1_vs_fp64"] = {
+                "state_l1rel": float(np.abs(a - b).sum() / np.abs(b).sum()),
+                "increment_l1rel": float(np.abs(a - b).sum() / np.abs(b - w0).sum()),
+                "density_max_rel": float(np.abs(ma[:, 0] - mb[:, 0]).max() / np.abs(mb[:, 0]).max()),
+                "energy_max_rel": float(np.abs(ma[:, 6] - mb[:, 6]).max() / np.abs(mb[:, 6]).max())}
+    if s:
+        sc = {k: v for k, v in s.items() if k not in ("i3", "W1")}
+        sc["value"] = round(sc["value"], 3)
+        sc["ms_per_step"] = round(sc["ms_per_step"], 3)
+        macs = 4096 * 9 * (4 * 64 + 4 * 64 * 64 + 64 * 4)           # per cell, unpadded channels
+        sc["cnn_roofline"] = {"bound": "mfma", "kernel": "k_conv_mfma (6 layers, 512 cells)",
+                              "achieved": round(2 * macs * Nx / (s["cnn_forward_ms_512_cells"] / 1e3) / 1e12, 2),
+                              "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s"}
+        line["surrogate"] = sc
     print(json.dumps(line), flush=True)
 
 

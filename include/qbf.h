@@ -140,10 +140,11 @@ int    qbf_cnn_adam(qbf_cnn *c, float lr, float b1, float b2, float eps);
  * FP64 moment correction of out - W (mask QBF_CONS_*, 0 = none).  Launches on e's stream. */
 int    qbf_surrogate_collision_step(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double dt,
 	float iscale, float oscale, double L, int mask, double *dOut);
-/* *residual = sum|C_net - C| / sum|C| over the cells hostIdx[0..nsample-1] of W (physical, device),
- * C = Cc + Cd computed exactly by e; nsample <= 64.  Synchronous. */
+/* Over the cells hostIdx[0..nsample-1] of W (physical, device), with C = Cc + Cd computed exactly by
+ * e and C_net the network's: sums[0] = sum|C_net - C|, sums[1] = sum|C|, sums[2] = sum|W|;
+ * nsample <= 64.  Synchronous. */
 int    qbf_surrogate_residual(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, const int *hostIdx,
-	int nsample, float iscale, float oscale, double *residual);
+	int nsample, float iscale, float oscale, double sums[3]);
 /* Strang step as qbf_inhom_step_periodic with the surrogate collision step; ncell <= maxBatch. */
 int    qbf_surrogate_inhom_step_periodic(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double L,
 	double h, double dt, float iscale, float oscale, int mask, double *dOut);

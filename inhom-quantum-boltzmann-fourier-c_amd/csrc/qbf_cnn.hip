@@ -87,65 +87,94 @@ __global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, i
 }
 
 // Implicit-GEMM 3x3 periodic convolution on the matrix cores.
-//   out[b][y][x][n] = sum_t sum_k in[b][y + DIR*(ky-1)][x + DIR*(kx-1)][k] * Wt[t][n][k]
+//   out[b][y][x][n] = sum_t sum_k in[b][y + DIR*d*(ky-1)][x + DIR*d*(kx-1)][k] * Wt[t][n][k]
 // DIR = +1: forward (Wt = W[t][cout][cin]); DIR = -1: backward data (Wt = W^T[t][cin][cout]).
 // MODE 0: out = bias + value, store pre-activation z (fp32) and GELU(z) (bf16 NHWC)
 // MODE 1: out = bias + value, store fp32 planar [B][4][S][S] (first 4 channels), times
 //         oscale (and a[b]^2 when amp is given)
 // MODE 2: backward data: g = value * gelu'(zprev) -> bf16 NHWC (gradient of the previous z)
+// A workgroup of CROWS waves computes CROWS image rows; the layer's weights are staged once in
+// LDS (row pitch KC + 4 elements: the 16 lanes of a 16-byte read cover all 64 banks but one
+// pair).  Each wave holds a 64-pixel x NC accumulator (4 x NC/16 MFMA tiles): per tap and
+// 32-channel block it loads 4 A fragments from global memory and NC/16 B fragments from LDS
+// for 4 * NC/16 MFMAs.
+constexpr int CROWS = 8;
 template <int KC, int NC, int DIR, int MODE>
-__global__ __launch_bounds__(256) void k_conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ Wt,
+__global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ Wt,
 	const float* __restrict__ bias, float* __restrict__ zout, bf16* __restrict__ aout, float* __restrict__ yout,
 	const float* __restrict__ zprev, float oscale, int dil, const float* __restrict__ amp)
 {
-	constexpr int NT = NC / 16;       // 16-wide output tiles
-	const int row = blockIdx.x;       // b * S + y
-	const int b = row / S, y = row % S;
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	constexpr int NT = NC / 16;       // 16-wide output-channel tiles
+	constexpr int MT = S / 16;        // 16-pixel tiles of a row
+	constexpr int KP = KC + 4;        // LDS pitch of a weight row (elements)
+	__shared__ __attribute__((aligned(16))) bf16 wl[9 * NC * KP];
+	const int b = blockIdx.x / (S / CROWS);
+	const int y = (blockIdx.x % (S / CROWS)) * CROWS + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
 	const int i16 = lane & 15, kq = lane >> 4;
-	const int x = wave * 16 + i16;    // pixel of this lane's A row
-	f32x4 acc[NT];
+	for (int v = threadIdx.x; v < 9 * NC * (KC / 8); v += 64 * CROWS) {
+		const int r = v / (KC / 8), c8 = v % (KC / 8);
+		const bf16x8 w = *(const bf16x8*)(Wt + (size_t)r * KC + c8 * 8);
+		bf16* d = wl + r * KP + c8 * 8;              // 8-byte aligned (KP = KC + 4)
+		*(uint2*)d = *(const uint2*)&w;
+		*(uint2*)(d + 4) = *((const uint2*)&w + 1);
+	}
+	__syncthreads();
+	f32x4 acc[MT][NT];
 #pragma unroll
-	for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+	for (int m = 0; m < MT; m++)
+#pragma unroll
+		for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 	for (int ky = 0; ky < 3; ky++) {
 		const int ys = (y + DIR * dil * (ky - 1) + 2 * S) & (S - 1);
+		const bf16* rowp = in + ((size_t)b * S + ys) * S * KC;
 		for (int kx = 0; kx < 3; kx++) {
-			const int xs = (x + DIR * dil * (kx - 1) + 2 * S) & (S - 1);
-			const bf16* src = in + (((size_t)b * S + ys) * S + xs) * KC;
-			const bf16* wt = Wt + (size_t)(ky * 3 + kx) * NC * KC;
-#pragma unroll
+			const bf16* wt = wl + (ky * 3 + kx) * NC * KP;
+			const int sh = DIR * dil * (kx - 1) + 2 * S;
+#pragma unroll 1
 			for (int kb = 0; kb < KC / 32; kb++) {
-				const bf16x8 a = *(const bf16x8*)(src + kb * 32 + kq * 8);
+				bf16x8 w[NT];
 #pragma unroll
 				for (int t = 0; t < NT; t++) {
-					const bf16x8 w = *(const bf16x8*)(wt + (size_t)(t * 16 + i16) * KC + kb * 32 + kq * 8);
-					acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[t], 0, 0, 0);
+					const bf16* q = wt + (t * 16 + i16) * KP + kb * 32 + kq * 8;
+					uint2 lo = *(const uint2*)q, hi = *(const uint2*)(q + 4);
+					uint4 u = {lo.x, lo.y, hi.x, hi.y};
+					w[t] = *(const bf16x8*)&u;
+				}
+#pragma unroll
+				for (int m = 0; m < MT; m++) {
+					const int xs = (m * 16 + i16 + sh) & (S - 1);
+					const bf16x8 a = *(const bf16x8*)(rowp + (size_t)xs * KC + kb * 32 + kq * 8);
+#pragma unroll
+					for (int t = 0; t < NT; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[t], acc[m][t], 0, 0, 0);
 				}
 			}
 		}
 	}
-	// lane holds D[4*kq + r][i16]: pixel wave*16 + 4*kq + r, channel t*16 + i16
+	// lane holds D[4*kq + r][i16] of tile (m, t): pixel m*16 + 4*kq + r, channel t*16 + i16
 	float osc = oscale;
 	if (MODE == 1 && amp) osc *= amp[b] * amp[b];
 #pragma unroll
-	for (int t = 0; t < NT; t++) {
-		const int n = t * 16 + i16;
+	for (int m = 0; m < MT; m++)
 #pragma unroll
-		for (int r = 0; r < 4; r++) {
-			const int xo = wave * 16 + kq * 4 + r;
-			const size_t pix = ((size_t)b * S + y) * S + xo;
-			const float v = acc[t][r];
-			if (MODE == 0) {
-				const float z = v + bias[n];
-				if (zout) zout[pix * NC + n] = z;
-				aout[pix * NC + n] = (bf16)gelu(z);
-			} else if (MODE == 1) {
-				if (n < CREAL) yout[((size_t)b * CREAL + n) * (S * S) + (size_t)y * S + xo] = osc * (v + bias[n]);
-			} else {
-				aout[pix * NC + n] = (bf16)(v * gelu_grad(zprev[pix * NC + n]));
+		for (int t = 0; t < NT; t++) {
+			const int n = t * 16 + i16;
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const int xo = m * 16 + kq * 4 + r;
+				const size_t pix = ((size_t)b * S + y) * S + xo;
+				const float v = acc[m][t][r];
+				if (MODE == 0) {
+					const float z = v + bias[n];
+					if (zout) zout[pix * NC + n] = z;
+					aout[pix * NC + n] = (bf16)gelu(z);
+				} else if (MODE == 1) {
+					if (n < CREAL) yout[((size_t)b * CREAL + n) * (S * S) + (size_t)y * S + xo] = osc * (v + bias[n]);
+				} else {
+					aout[pix * NC + n] = (bf16)(v * gelu_grad(zprev[pix * NC + n]));
+				}
 			}
 		}
-	}
 }
 
 // Weight and bias gradients on the VALU:
@@ -307,33 +336,39 @@ __global__ void k_gather_cells(const double* __restrict__ W, const int* __restri
 	out[i] = W[(size_t)idx[j] * CELL + e];
 }
 
-// res[0] = sum |a - b|, res[1] = sum |b|  (a fp32, b fp64; one workgroup of 1024 threads)
+// res[0] = sum |a - b|, res[1] = sum |b|, res[2] = sum |w|  (a fp32, b and w fp64; one workgroup
+// of 1024 threads)
 __global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ a, const double* __restrict__ b,
-	long long n, double* __restrict__ res)
+	const double* __restrict__ w, long long n, double* __restrict__ res)
 {
-	__shared__ double sh[2][16];
-	double d = 0, m = 0;
+	__shared__ double sh[3][16];
+	double d = 0, m = 0, s = 0;
 	for (long long i = threadIdx.x; i < n; i += blockDim.x) {
 		d += fabs((double)a[i] - b[i]);
 		m += fabs(b[i]);
+		s += fabs(w[i]);
 	}
 	for (int o = 32; o > 0; o >>= 1) {
 		d += __shfl_xor(d, o);
 		m += __shfl_xor(m, o);
+		s += __shfl_xor(s, o);
 	}
 	if ((threadIdx.x & 63) == 0) {
 		sh[0][threadIdx.x >> 6] = d;
 		sh[1][threadIdx.x >> 6] = m;
+		sh[2][threadIdx.x >> 6] = s;
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		double D = 0, M = 0;
-		for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-			D += sh[0][w];
-			M += sh[1][w];
+		double D = 0, M = 0, W = 0;
+		for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+			D += sh[0][k];
+			M += sh[1][k];
+			W += sh[2][k];
 		}
 		res[0] = D;
 		res[1] = M;
+		res[2] = W;
 	}
 }
 
@@ -375,7 +410,7 @@ struct qbf_cnn {
 	static constexpr int RMAX = 64;       // residual check: at most RMAX sampled cells
 	int* sidx = nullptr;                  // sampled cell indices (device)
 	double *swf = nullptr, *swc = nullptr; // Fourier state / exact collision of the sample (complex)
-	double* sres = nullptr;               // two partial sums of the residual
+	double* sres = nullptr;               // three sums of the residual
 	int step = 0;
 	std::vector<void*> allocs;
 	float* alloc_f(size_t n)
@@ -445,7 +480,7 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		if (hipMalloc(&p1, qbf_cnn::RMAX * sizeof(int)) == hipSuccess) c->allocs.push_back(p1); else p1 = nullptr;
 		if (hipMalloc(&p2, cplx) == hipSuccess) c->allocs.push_back(p2); else p2 = nullptr;
 		if (hipMalloc(&p3, cplx) == hipSuccess) c->allocs.push_back(p3); else p3 = nullptr;
-		if (hipMalloc(&p4, 2 * sizeof(double)) == hipSuccess) c->allocs.push_back(p4); else p4 = nullptr;
+		if (hipMalloc(&p4, 3 * sizeof(double)) == hipSuccess) c->allocs.push_back(p4); else p4 = nullptr;
 		c->sidx = (int*)p1;
 		c->swf = (double*)p2;
 		c->swc = (double*)p3;
@@ -544,7 +579,7 @@ static int run_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float*
 	if (c->norm) hipLaunchKernelGGL(k_cell_amp, dim3(B), dim3(256), 0, c->stream, dIn, c->amp);
 	hipLaunchKernelGGL(k_to_nhwc, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B, iscale,
 		amp);
-	const dim3 grid(B * S), blk(256);
+	const dim3 grid(B * (S / CROWS)), blk(64 * CROWS);
 	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
 		train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f, c->dil[0], nullptr);
 	for (int l = 1; l < NL - 1; l++)
@@ -579,18 +614,18 @@ int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B,
 	const float inv_n = 1.0f / (float)(npx * CREAL);
 	hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dOut, dTarget,
 		c->g[NL - 1], c->loss, B, inv_n, oscale, c->norm ? c->amp : nullptr);
-	const dim3 grid(B * S), blk(256);
+	const dim3 grid(B * (S / CROWS)), cblk(64 * CROWS), blk(256);
 	const int rows = 16;
 	const dim3 wgrid(B * (S / rows));
 	// last layer: weight gradient, then data gradient into the last hidden pre-activation
 	hipLaunchKernelGGL((k_conv_wgrad<CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gw[NL - 1],
 		c->gb[NL - 1], rows, c->dil[NL - 1]);
-	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, -1, 2>), grid, blk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
+	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
 		nullptr, c->g[NL - 2], nullptr, c->z[NL - 2], 1.0f, c->dil[NL - 1], nullptr);
 	for (int l = NL - 2; l >= 1; l--) {
 		hipLaunchKernelGGL((k_conv_wgrad<CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows,
 			c->dil[l]);
-		hipLaunchKernelGGL((k_conv_mfma<CH, CH, -1, 2>), grid, blk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
+		hipLaunchKernelGGL((k_conv_mfma<CH, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
 			c->g[l - 1], nullptr, c->z[l - 1], 1.0f, c->dil[l], nullptr);
 	}
 	hipLaunchKernelGGL((k_conv_wgrad<CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows,
@@ -657,13 +692,15 @@ int qbf_surrogate_collision_step(qbf_cnn* c, qbf_engine* e, const double* dW, in
 }
 
 // Residual of the surrogate on sampled cells of physical states W: the exact collision
-// operator C = Cc + Cd of the engine (any precision) against oscale * net(iscale * W),
-//   *residual = sum |C_net - C| / sum |C|  over the nsample cells hostIdx[0..nsample-1].
-// Synchronous (the result is read back to the host).
+// operator C = Cc + Cd of the engine (any precision) against C_net = oscale * net(iscale * W),
+// summed over the nsample cells hostIdx[0..nsample-1]:
+//   sums[0] = sum |C_net - C|,  sums[1] = sum |C|,  sums[2] = sum |W|.
+// sums[0] / sums[1] is the relative error of the operator; dt * sums[0] / sums[2] the relative
+// error one surrogate step adds to the state.  Synchronous (the sums are read back to the host).
 int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncell, const int* hostIdx, int nsample,
-	float iscale, float oscale, double* residual)
+	float iscale, float oscale, double* sums)
 {
-	if (!c || !e || !dW || !hostIdx || !residual || nsample < 1 || nsample > qbf_cnn::RMAX || nsample > c->maxB)
+	if (!c || !e || !dW || !hostIdx || !sums || nsample < 1 || nsample > qbf_cnn::RMAX || nsample > c->maxB)
 		return cnn_err(-1, "qbf_surrogate_residual: bad argument (1 <= nsample <= 64 and <= maxBatch)");
 	if (qbf_engine_N(e) != S) return cnn_err(-1, "qbf_surrogate_residual: engine grid must be 64");
 	for (int j = 0; j < nsample; j++)
@@ -672,7 +709,7 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 	c->stream = (hipStream_t)qbf_get_stream(e);
 	int r = 0;
 	const long long n = (long long)nsample * CREAL * S * S;
-	double h[2] = {0, 0};
+	double h[3] = {0, 0, 0};
 	if (hipMemcpyAsync(c->sidx, hostIdx, nsample * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
 		r = cnn_err(-2, "index upload failed");
 	if (!r) {
@@ -688,13 +725,14 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 		r = run_forward(c, c->sx, nsample, iscale, c->sc, oscale, false);
 	}
 	if (!r) {
-		hipLaunchKernelGGL(k_l1_residual, dim3(1), dim3(1024), 0, c->stream, c->sc, c->sw2, n, c->sres);
+		hipLaunchKernelGGL(k_l1_residual, dim3(1), dim3(1024), 0, c->stream, c->sc, c->sw2, c->sw, n, c->sres);
 		if (hipMemcpyAsync(h, c->sres, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
 			hipStreamSynchronize(c->stream) != hipSuccess)
 			r = cnn_err(-2, "residual read-back failed");
 	}
 	c->stream = saved;
-	if (!r) *residual = h[1] > 0 ? h[0] / h[1] : (h[0] > 0 ? INFINITY : 0.0);
+	if (!r)
+		for (int k = 0; k < 3; k++) sums[k] = h[k];
 	return r;
 }
 

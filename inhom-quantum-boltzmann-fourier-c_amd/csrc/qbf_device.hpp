@@ -218,15 +218,16 @@ __device__ __forceinline__ cx<R> pin(cx<R> w)
 }
 
 // Forward FFT of one line: interleaved layout in, blocked layout out.
-// twP[j] = (cos, sin)(2 pi j / P), j < P; twP[P + p*E + u] = (cos, sin)(2 pi p u / P) (in LDS).
+// twP[j] = (cos, sin)(2 pi j / P), j < P; twP[P + u*G + p] = (cos, sin)(2 pi p u / P) (in LDS; the
+// lanes of one read hit G consecutive entries, so distinct lanes never share a bank).
 template <typename R, int P, int G, int E>
 __device__ __forceinline__ void line_fwd(cx<R>* v, int p, const cx<R>* twP)
 {
 	dft<R, E, -1>(v);
 	if constexpr (G > 1) {
-		const cx<R>* tl = twP + P + p * E + opaque_zero();
+		const cx<R>* tl = twP + P + p + opaque_zero();
 #pragma unroll
-		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], tl[u]);
+		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], tl[u * G]);
 #pragma unroll
 		for (int h = G / 2; h >= 1; h >>= 1) {
 			// DIF butterfly across lanes p, p^h:  lo: a + b,  hi: (a - b) * conj(w)
@@ -281,9 +282,9 @@ __device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
 				}
 			}
 		}
-		const cx<R>* tl = twP + P + p * E + opaque_zero();
+		const cx<R>* tl = twP + P + p + opaque_zero();
 #pragma unroll
-		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], tl[u]);
+		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], tl[u * G]);
 	}
 	dft<R, E, +1>(v);
 }
@@ -335,13 +336,13 @@ struct Xpose {
 };
 
 // Fill the LDS twiddle tables from a global table of (cos, sin)(2 pi j / 256) in double:
-// twP[j] = w^j for j < P, and the per-lane table twP[P + p*E + u] = w^(p*u), w = exp(2 pi i / P).
+// twP[j] = w^j for j < P, and the per-lane table twP[P + u*G + p] = w^(p*u), w = exp(2 pi i / P).
 template <typename R, int P, int T>
 __device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
 {
 	constexpr int G = T / P, E = P / G;
 	for (int j = threadIdx.x; j < 2 * P; j += T) {
-		const int k = j < P ? j : ((j - P) / E) * ((j - P) % E);
+		const int k = j < P ? j : ((j - P) % G) * ((j - P) / G);
 		const double2 w = tw256[k * (256 / P)];
 		twP[j] = mk<R>((R)w.x, (R)w.y);
 	}

@@ -56,7 +56,10 @@ template <> struct Cfg<double, 32> { static constexpr int T = 256; static conste
 template <> struct Cfg<double, 64> { static constexpr int T = 1024; static constexpr bool XG = true; };
 template <> struct Cfg<float, 16> { static constexpr int T = 128; static constexpr bool XG = false; };
 template <> struct Cfg<float, 32> { static constexpr int T = 256; static constexpr bool XG = false; };
-template <> struct Cfg<float, 64> { static constexpr int T = 1024; static constexpr bool XG = false; };
+#ifndef QBF_F32_N64_THREADS
+#define QBF_F32_N64_THREADS 1024   // 512 also fits in registers (G = 4, E = 32)
+#endif
+template <> struct Cfg<float, 64> { static constexpr int T = QBF_F32_N64_THREADS; static constexpr bool XG = false; };
 template <int N> struct CfgNN;
 template <> struct CfgNN<16> { static constexpr int T = 64; };
 template <> struct CfgNN<32> { static constexpr int T = 128; };

@@ -18,7 +18,8 @@ import subprocess
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libqbf.so")
+# QBF_LIB_PATH selects another build of the library (for example one with other kernel shapes)
+LIB_PATH = os.environ.get("QBF_LIB_PATH") or os.path.join(PKG_DIR, "lib", "libqbf.so")
 
 FP64, FP32 = 0, 1
 CC, CD, CFULL = 1, 2, 3

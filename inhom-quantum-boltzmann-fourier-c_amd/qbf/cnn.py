@@ -159,15 +159,14 @@ class CNN:
         return out
 
     def residual(self, engine, W, cells, iscale, oscale):
-        """sum|C_net - C| / sum|C| over the given cells of W (float64 cuda [ncell, 4, 64, 64]), with
-        C the engine's exact collision operator."""
+        """(sum|C_net - C|, sum|C|, sum|W|) over the given cells of W (float64 cuda
+        [ncell, 4, 64, 64]), with C the engine's exact collision operator."""
         engine._bind()
         idx = np.ascontiguousarray(cells, np.int32)
-        r = C.c_double()
+        r = (C.c_double * 3)()
         _check(_lib().qbf_surrogate_residual(self._h, engine._h, _ptrd(W), W.shape[0],
-                                             idx.ctypes.data_as(C.POINTER(C.c_int)), idx.size, iscale, oscale,
-                                             C.byref(r)))
-        return r.value
+                                             idx.ctypes.data_as(C.POINTER(C.c_int)), idx.size, iscale, oscale, r))
+        return r[0], r[1], r[2]
 
     def inhom_step_periodic(self, engine, W, L, h, dt, iscale, oscale, mask=0, out=None):
         """Strang step (half transport, surrogate collision step, half transport)."""

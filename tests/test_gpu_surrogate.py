@@ -57,13 +57,14 @@ def test_residual_matches_manual(setup):
     qbf, S, wl, e, net, W, torch = setup
     isc, osc = 2.0, 0.05
     cells = np.array([3, 17, 40])
-    r = net.residual(e, W, cells, isc, osc)
+    d, c, w = net.residual(e, W, cells, isc, osc)
     Ws = W[torch.from_numpy(cells).cuda()].contiguous()
     Cex = S.exact_collision_phys(e, Ws)
     Cn = net.forward(Ws.float().contiguous(), isc, osc).double()
-    want = float((Cn - Cex).abs().sum() / Cex.abs().sum())
-    print("residual", r, "manual", want)
-    assert abs(r - want) <= 1e-9 * want
+    want = (float((Cn - Cex).abs().sum()), float(Cex.abs().sum()), float(Ws.abs().sum()))
+    print("residual sums", (d, c, w), "manual", want)
+    for got, exp in zip((d, c, w), want):
+        assert abs(got - exp) <= 1e-9 * exp
     with pytest.raises(qbf.QBFError):
         net.residual(e, W, np.array([64]), isc, osc)
 
