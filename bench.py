@@ -205,9 +205,10 @@ def main():
     if "surrogate" in paths:
         from qbf import surrogate as S
         t0 = time.perf_counter()
-        X, Y = S.training_set(e32, seeds=range(100, 116), times=np.linspace(0.0, 0.06, 7), cells_per_state=64, Nx=Nx)
+        X, Y = S.training_set(e32, seeds=range(100, 116), times=np.linspace(0.0, 0.04, 9), cells_per_state=64, Nx=Nx)
         Xv, Yv = S.training_set(e32, seeds=[args.seed], times=[0.0, 0.03], cells_per_state=32, Nx=Nx, rng_seed=1)
         isc, osc = S.scales(X, Y)
+        n_train = int(X.shape[0])
         torch.cuda.synchronize()
         t_data = time.perf_counter() - t0
         net = S.make_net(Nx, seed=0)
@@ -250,7 +251,7 @@ def main():
             "residual_median": float(np.median(st.residuals)) if st.residuals else None,
             "residual_max": float(np.max(st.residuals)) if st.residuals else None,
             "operator_residual_median": float(np.median(st.op_residuals)) if st.op_residuals else None,
-            "training": {"samples": int(16 * 7 * 64), "steps": args.train_steps, "data_s": round(t_data, 2),
+            "training": {"samples": n_train, "steps": args.train_steps, "data_s": round(t_data, 2),
                          "train_s": round(t_train, 2), "final_loss": float(hist[-1]),
                          "val_rel_l1_seed3": round(val, 5)},
             "cnn_forward_ms_512_cells": round(cnn_ms, 3)}

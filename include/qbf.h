@@ -97,6 +97,12 @@ enum { QBF_CONS_MASS = 1, QBF_CONS_MOMENTUM = 2, QBF_CONS_ENERGY = 4 };
 int qbf_set_moment_correction(qbf_engine *e, int mask);
 int qbf_moment_correct(qbf_engine *e, const double *dWold, double *dWnew, int ncell, double L, int mask);
 
+/* Per-cell diagnostics of physical states [ncell][4][N][N] into dOut[ncell][10] (device):
+ * mass and spin densities (4), momentum (2), energy, entropy sum_v [s(l+) + s(l-)] dv with the binary
+ * entropy s and the eigenvalues l+- = w0 +- |w| of w0 I + w.sigma, and the min and max eigenvalue
+ * (visualization/wignerMoments.m:15-37, entropy.m, visualizeHomBoltzmann.m:26-45,98-105). */
+int qbf_cell_diagnostics(qbf_engine *e, const double *dW, int ncell, double L, double *dOut);
+
 /* Timing of the dominant kernel (the I3 K-operator kernel): qbf_profile(e, 1) clears and
  * starts recording HIP events around each of its launches on the engine stream;
  * qbf_profile_read returns the summed kernel time, the launch count and the number of

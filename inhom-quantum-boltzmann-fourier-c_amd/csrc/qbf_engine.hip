@@ -800,6 +800,14 @@ int qbf_set_moment_correction(qbf_engine* e, int mask)
 	return 0;
 }
 
+int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, double* dOut)
+{
+	if (!e || !dW || !dOut || ncell < 1) return set_err(-1, "qbf_cell_diagnostics: bad argument");
+	hipLaunchKernelGGL(k_cell_diag, dim3(ncell), dim3(256), 0, e->stream, dW, e->gridN, L, dOut);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
 int qbf_moment_correct(qbf_engine* e, const double* Wold, double* Wnew, int ncell, double L, int mask)
 {
 	if (!e || !Wold || !Wnew || ncell < 1 || mask < 0 || mask > 7) return set_err(-1, "bad argument");

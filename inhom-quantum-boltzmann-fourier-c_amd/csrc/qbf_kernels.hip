@@ -544,6 +544,64 @@ __global__ __launch_bounds__(256) void k_moment_correct(const double* __restrict
 	}
 }
 
+// Per-cell diagnostics of physical states [ncell][4][N][N] (one workgroup of 256 per cell):
+//   out[0..3]  sum_v w_c dv                    (mass and spin densities)
+//   out[4..5]  sum_v v_x w0 dv, sum_v v_y w0 dv  (momentum)
+//   out[6]     sum_v |v|^2/2 w0 dv             (energy)
+//   out[7]     sum_v [s(l+) + s(l-)] dv,  s(p) = -(p ln p + (1-p) ln(1-p)) on (0, 1), else 0
+//   out[8], out[9]  min and max eigenvalue over v
+// with the eigenvalues l+- = w0 +- |(w1, w2, w3)| of the 2x2 matrix w0 I + w . sigma, dv = (2L/N)^2
+// (visualization/wignerMoments.m:15-37, entropy.m, visualizeHomBoltzmann.m:26-45, 98-105).
+__device__ __forceinline__ double bin_entropy(double p)
+{
+	return (p > 0.0 && p < 1.0) ? -(p * log(p) + (1.0 - p) * log1p(-p)) : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_cell_diag(const double* __restrict__ W, int N, double L,
+	double* __restrict__ out)
+{
+	__shared__ double sh[4];
+	const int NN = N * N;
+	const double* w = W + (size_t)blockIdx.x * 4 * NN;
+	const double sc = (2 * L) / N;
+	double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	double lmin = INFINITY, lmax = -INFINITY;
+	for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+		const int l = e / N, k = e % N;
+		const double vx = l < N / 2 ? sc * l : sc * (l - N);
+		const double vy = k < N / 2 ? sc * k : sc * (k - N);
+		const double w0 = w[e], w1 = w[NN + e], w2 = w[2 * NN + e], w3 = w[3 * NN + e];
+		acc[0] += w0;
+		acc[1] += w1;
+		acc[2] += w2;
+		acc[3] += w3;
+		acc[4] += vx * w0;
+		acc[5] += vy * w0;
+		acc[6] += 0.5 * (vx * vx + vy * vy) * w0;
+		const double r = sqrt(w1 * w1 + w2 * w2 + w3 * w3);
+		acc[7] += bin_entropy(w0 + r) + bin_entropy(w0 - r);
+		lmin = fmin(lmin, w0 - r);
+		lmax = fmax(lmax, w0 + r);
+	}
+	const double dv = sc * sc;
+	for (int i = 0; i < 8; i++) {
+		const double t = block_sum_256(acc[i], sh);
+		if (threadIdx.x == 0) out[(size_t)blockIdx.x * 10 + i] = t * dv;
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		lmin = fmin(lmin, __shfl_xor(lmin, o));
+		lmax = fmax(lmax, __shfl_xor(lmax, o));
+	}
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = lmin;
+	__syncthreads();
+	if (threadIdx.x == 0) out[(size_t)blockIdx.x * 10 + 8] = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = lmax;
+	__syncthreads();
+	if (threadIdx.x == 0) out[(size_t)blockIdx.x * 10 + 9] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
 // y = a * x + b * y  (elementwise, double)
 __global__ void k_axpby(const double* __restrict__ x, double* __restrict__ y, double a, double b, long long n)
 {

@@ -79,6 +79,7 @@ def lib():
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
             "qbf_set_moment_correction": (i, [vp, i]),
             "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
+            "qbf_cell_diagnostics": (i, [vp, vp, i, d, vp]),
             "qbf_profile": (i, [vp, i]),
             "qbf_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]),
             "qbf_legendre_gauss": (None, [u, d, d, _dp, _dp]),
@@ -177,6 +178,14 @@ class Engine:
         self._bind()
         _check(lib().qbf_moment_correct(self._h, _ptr(Wold), _ptr(Wnew), Wold.shape[0], L, mask))
         return Wnew
+
+    def cell_diagnostics(self, W, L):
+        """[ncell, 10] float64 cuda tensor: mass, spin x/y/z, momentum x/y, energy, entropy,
+        min and max eigenvalue per cell of physical states W."""
+        self._bind()
+        out = self._torch.empty((W.shape[0], 10), dtype=self._torch.float64, device=W.device)
+        _check(lib().qbf_cell_diagnostics(self._h, _ptr(W), W.shape[0], L, _ptr(out)))
+        return out
 
     # --------------------------------------------------------------- stream
     def _bind(self):
