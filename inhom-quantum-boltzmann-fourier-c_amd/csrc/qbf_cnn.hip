@@ -218,11 +218,13 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, 
 				const float g = zs[px][co];
 				bacc += g;
 #pragma unroll
-				for (int ky = 0; ky < 3; ky++)
+				for (int kx = 0; kx < 3; kx++) {
+					const int pxs = (px + dil * (kx - 1) + 2 * S) & (S - 1);
 #pragma unroll
-					for (int kx = 0; kx < 3; kx++)
+					for (int ky = 0; ky < 3; ky++)
 #pragma unroll
-						for (int j = 0; j < 16; j++) acc[ky * 3 + kx][j] += g * xs[ky][px + kx][cb * 16 + j];
+						for (int j = 0; j < 16; j++) acc[ky * 3 + kx][j] += g * xs[ky][pxs][cb * 16 + j];
+				}
 			}
 		}
 	}
