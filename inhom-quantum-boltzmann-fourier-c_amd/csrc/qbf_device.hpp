@@ -188,6 +188,16 @@ struct Geo {
 	static_assert(G <= 64 && (64 % G) == 0, "a line must sit inside one wavefront");
 };
 
+// Logical block index that keeps consecutive logical blocks on one XCD.  The dispatcher hands
+// block b to XCD b % 8, so XCD x runs the logical blocks [x n/8, (x+1) n/8): workgroups that read
+// the same grids (the outputs of one cell) share that XCD's L2.  A permutation of [0, n) for any
+// n divisible by 8 (identity otherwise), so correctness never depends on the dispatch order.
+__device__ __forceinline__ int xcd_block()
+{
+	const int n = gridDim.x, b = blockIdx.x;
+	return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // An SGPR zero the compiler cannot see through: adding it to an LDS index keeps table loads
 // next to their use instead of hoisting them (and their registers) out of the term loops.
 __device__ __forceinline__ int opaque_zero()

@@ -122,8 +122,9 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
-	const int inst = blockIdx.x % a.ninst;
-	const int cell = blockIdx.x / a.ninst;
+	const int lb = xcd_block();
+	const int inst = lb % a.ninst;
+	const int cell = lb / a.ninst;
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instSrc[inst]) * (N * N);
 	const cx<R>* Fh = (const cx<R>*)a.Fhat + ((size_t)cell * a.nFh + a.instFh[inst]) * (P * P);
@@ -164,8 +165,9 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
-	const int inst = blockIdx.x % a.ninst;
-	const int cell = blockIdx.x / a.ninst;
+	const int lb = xcd_block();
+	const int inst = lb % a.ninst;
+	const int cell = lb / a.ninst;
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const double2* f = a.src + ((size_t)cell * a.nsrc + a.instF[inst]) * (N * N);
 	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instG[inst]) * (N * N);
@@ -197,9 +199,10 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 	__shared__ cx<R> tw[2 * P];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
-	const int chunk = blockIdx.x % a.nchunk;
-	const int out = (blockIdx.x / a.nchunk) % a.nout;
-	const int cell = blockIdx.x / (a.nchunk * a.nout);
+	const int lb = xcd_block();
+	const int chunk = lb % a.nchunk;
+	const int out = (lb / a.nchunk) % a.nout;
+	const int cell = lb / (a.nchunk * a.nout);
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const int t = threadIdx.x;
 	const int qa = (int)(((long long)chunk * a.nq) / a.nchunk);
