@@ -38,10 +38,13 @@ METRIC = "collision evals/s at N=64,Nx=512: FP32/surrogate vs FP64 plain, ≤1e-
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "bf16": 2.5e15}
 REF_FILE = os.path.join(ROOT, "tests", "golden", "periodic64_fp64_ref.npz")
 
-# HBM traffic of one k_i3acc<float, 64, 1024> launch of this workload (FETCH_SIZE + WRITE_SIZE,
-# rocprofv3 --pmc on `bench.py --no-cpu-baseline --paths mixed --mixed-steps 1 --warmup 0
-# --copy-calibration`, each counter scaled by its reading for a 256 MiB device copy).
-I3_TRAFFIC_BYTES = None
+# HBM traffic of one k_i3acc<float, 64, 1024> launch of this workload (8 quadrature terms x 512
+# cells x 16 K-operator applications): FETCH_SIZE 4.92e6 KiB x 2.000 + WRITE_SIZE 2.62e5 KiB x 1.000,
+# from rocprofv3 --pmc passes over `bench.py --no-cpu-baseline --paths mixed --mixed-steps 1
+# --warmup 0 --copy-calibration`, each counter scaled by its reading for the 256 MiB device copy
+# of the same run.  Algorithmic bytes of the launch: 2.77e9 (F grids 2 GiB, transformed state
+# 256 MiB, accumulator read + write 512 MiB, psi2 0.5 MiB).
+I3_TRAFFIC_BYTES = 10.34e9
 
 
 def cpu_baseline(N, J, M, cores, seed):
