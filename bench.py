@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--paths", default="surrogate,mixed,fp64")
     ap.add_argument("--mixed-steps", type=int, default=3, help="timed steps of the FP32 exact path")
     ap.add_argument("--fp64-steps", type=int, default=1, help="timed FP64 plain steps")
-    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--exact-every", type=int, default=10)
     ap.add_argument("--sample-frac", type=float, default=0.02)
     ap.add_argument("--tol", type=float, default=1e-3,
@@ -210,7 +210,7 @@ def main():
         t0 = time.perf_counter()
         X, Y = S.training_set(e32, seeds=range(100, 116), times=np.linspace(0.0, 0.04, 9), cells_per_state=64, Nx=Nx)
         Xv, Yv = S.training_set(e32, seeds=[args.seed], times=[0.0, 0.03], cells_per_state=32, Nx=Nx, rng_seed=1)
-        isc, osc = S.scales(X, Y)
+        isc, osc = S.scales(X, Y, norm=False)
         n_train = int(X.shape[0])
         torch.cuda.synchronize()
         t_data = time.perf_counter() - t0

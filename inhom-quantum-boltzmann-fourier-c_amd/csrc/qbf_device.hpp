@@ -227,17 +227,26 @@ __device__ __forceinline__ cx<R> pin(cx<R> w)
 	return w;
 }
 
+// Layout of the per-lane twiddle table w^(p u): float grids interleave it by lane, [u][p], so
+// that the G distinct addresses of one read fall into distinct banks; double grids keep it
+// [p][u] (their N = 64 kernels are register-bound, and the interleaved addressing costs them
+// registers).
+template <typename R, int G, int E>
+__device__ __forceinline__ constexpr int tw_lane(int p) { return sizeof(R) == 4 ? p : p * E; }
+template <typename R, int G>
+__device__ __forceinline__ constexpr int tw_step() { return sizeof(R) == 4 ? G : 1; }
+
 // Forward FFT of one line: interleaved layout in, blocked layout out.
-// twP[j] = (cos, sin)(2 pi j / P), j < P; twP[P + u*G + p] = (cos, sin)(2 pi p u / P) (in LDS; the
-// lanes of one read hit G consecutive entries, so distinct lanes never share a bank).
+// twP[j] = (cos, sin)(2 pi j / P), j < P; the per-lane table (cos, sin)(2 pi p u / P) follows it
+// (in LDS, laid out as tw_lane / tw_step below).
 template <typename R, int P, int G, int E>
 __device__ __forceinline__ void line_fwd(cx<R>* v, int p, const cx<R>* twP)
 {
 	dft<R, E, -1>(v);
 	if constexpr (G > 1) {
-		const cx<R>* tl = twP + P + p + opaque_zero();
+		const cx<R>* tl = twP + P + tw_lane<R, G, E>(p) + opaque_zero();
 #pragma unroll
-		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], tl[u * G]);
+		for (int u = 1; u < E; u++) v[u] = cmulc<R>(v[u], tl[tw_step<R, G>() * u]);
 #pragma unroll
 		for (int h = G / 2; h >= 1; h >>= 1) {
 			// DIF butterfly across lanes p, p^h:  lo: a + b,  hi: (a - b) * conj(w)
@@ -292,22 +301,24 @@ __device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
 				}
 			}
 		}
-		const cx<R>* tl = twP + P + p + opaque_zero();
+		const cx<R>* tl = twP + P + tw_lane<R, G, E>(p) + opaque_zero();
 #pragma unroll
-		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], tl[u * G]);
+		for (int u = 1; u < E; u++) v[u] = cmul<R>(v[u], tl[tw_step<R, G>() * u]);
 	}
 	dft<R, E, +1>(v);
 }
 
 // LDS exchanges.  The grid is stored row-major with pitch P + G and a skew of SKEW elements
-// per block of E rows.  With 64 banks of 4 bytes one access phase serves LANES lanes
-// (256 bytes).  Writes / reads of rows (R_i) hit LANES/G consecutive rows times G
-// consecutive columns: distinct banks because the pitch is G modulo LANES.  Writes / reads
-// of columns (C_b) hit LANES/G consecutive columns in G row blocks E rows apart: E*pitch is
-// a multiple of LANES, and the per-block skew LANES/G separates the blocks.
+// per block of E rows.  One access phase of the gfx950 LDS serves 512 bytes: LANES lanes
+// of one element each, conflict-free when their addresses are distinct modulo LANES elements
+// (measured: lane-linear 8-byte accesses run without bank conflicts, stride-2 accesses with
+// 2-way conflicts).  Writes / reads of rows (R_i) hit LANES/G consecutive rows times G
+// consecutive columns: distinct because the pitch is G modulo LANES.  Writes / reads of
+// columns (C_b) hit LANES/G consecutive columns in G row blocks E rows apart: E*pitch is a
+// multiple of LANES, and the per-block skew LANES/G separates the blocks.
 template <typename R, class GE>
 struct Xpose {
-	static constexpr int LANES = 256 / (int)sizeof(cx<R>);
+	static constexpr int LANES = 512 / (int)sizeof(cx<R>);
 	static constexpr int SKEW = LANES / GE::G;
 	static constexpr int SIZE = GE::P * GE::PITCH + GE::G * SKEW;
 	cx<R>* g0;
@@ -346,13 +357,14 @@ struct Xpose {
 };
 
 // Fill the LDS twiddle tables from a global table of (cos, sin)(2 pi j / 256) in double:
-// twP[j] = w^j for j < P, and the per-lane table twP[P + u*G + p] = w^(p*u), w = exp(2 pi i / P).
+// twP[j] = w^j for j < P, and the per-lane table w^(p*u) at twP[P + tw_lane(p) + tw_step * u],
+// w = exp(2 pi i / P).
 template <typename R, int P, int T>
 __device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
 {
 	constexpr int G = T / P, E = P / G;
 	for (int j = threadIdx.x; j < 2 * P; j += T) {
-		const int k = j < P ? j : ((j - P) % G) * ((j - P) / G);
+		const int k = j < P ? j : (sizeof(R) == 4 ? ((j - P) % G) * ((j - P) / G) : ((j - P) / E) * ((j - P) % E));
 		const double2 w = tw256[k * (256 / P)];
 		twP[j] = mk<R>((R)w.x, (R)w.y);
 	}

@@ -44,12 +44,14 @@ def make_engine(N=N_SURR, max_cells=512, J=wl.J_ANG, M=wl.M_GL, precision=FP32):
     return e
 
 
-DILATION = (1, 2, 4, 8, 4, 1)      # receptive field 43 x 43 pixels: the collision ball has ~40
+DILATION = (1, 2, 4, 8, 4, 1)      # receptive field 43 x 43 pixels (an option of the kernels)
 
 
 def make_net(max_batch, seed=0):
-    """The surrogate network: dilated, with per-cell normalisation."""
-    return CNN(max_batch=max_batch, seed=seed, dil=DILATION, norm=True)
+    """The surrogate network: six undilated 3x3 layers without per-cell normalisation.  (Measured
+    at N=64, Nx=512 over 40 bare surrogate steps: the dilated and the normalised variants fit the
+    operator as well but drift 2-30x more in energy.)"""
+    return CNN(max_batch=max_batch, seed=seed)
 
 
 def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, rng_seed=0):
@@ -70,7 +72,7 @@ def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, r
     return Xd.float().contiguous(), Yd.float().contiguous()
 
 
-def scales(X, Y, norm=True):
+def scales(X, Y, norm=False):
     """iscale maps the states to O(1); oscale the collision operator (per unit a^2 with the
     per-cell normalisation, a = max |W0| of the cell)."""
     if not norm:
