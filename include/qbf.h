@@ -87,6 +87,14 @@ int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A
 int qbf_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
 	const double *dBext, double *dWnext);
 
+/* numsteps frames of the periodic problem from dW0 (device), the state kept on the device
+ * (src/simulation.c:1288-1301): frame 0 = W0, frame it = qbf_inhom_step_periodic of frame it-1
+ * (with the engine's conservation setting).  Frames 0, stride, 2 stride, ... < numsteps are
+ * copied to hostFrames (host, (numsteps - 1) / stride + 1 frames of ncell cells); the call
+ * returns when they are there.  stride = 1 gives the reference's Wevolv. */
+int qbf_simulate_periodic(qbf_engine *e, const double *dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double *dBext, double *hostFrames);
+
 /* FP64 conservation correction of a collision step (physical states, [ncell][4][N][N]):
  * Wnew is replaced by Wold + P(Wnew - Wold), where P removes, per cell, the selected moments
  * of the increment with a |w0|-weighted least-squares correction.  mask: QBF_CONS_MASS (mass

@@ -489,20 +489,12 @@ void SimulationInhomogeneousPeriodic(const wignerV_t *restrict_ W0, const quadI1
 	if (numsteps < 2) return;
 	if (!engine((int)nv) || use_all(quadI1, quadI2, quadI3, quadI4)) return;
 	const size_t sz = nv * sizeof(wignerV_t);
-	double *dA = dbuf(5, sz), *dBuf = dbuf(6, sz), *dB = dbuf(2, 3 * nv * sizeof(double));
-	if (!dA || !dBuf || !dB) return;
-	if (qbf_h2d(g_eng, dA, W0, sz) || qbf_h2d(g_eng, dB, Bext, 3 * nv * sizeof(double))) {
+	double *dA = dbuf(5, sz), *dB = dbuf(2, 3 * nv * sizeof(double));
+	if (!dA || !dB) return;
+	/* the state stays on the GPU; every frame is copied out (src/simulation.c:1288-1301) */
+	if (qbf_h2d(g_eng, dA, W0, sz) || qbf_h2d(g_eng, dB, Bext, 3 * nv * sizeof(double)) ||
+		qbf_simulate_periodic(g_eng, dA, (int)nv, interm->L, h, dt, numsteps, 1, dB, (double *)Wevolv))
 		report("SimulationInhomogeneousPeriodic");
-		return;
-	}
-	for (unsigned it = 1; it < numsteps; it++) {
-		double *src = (it & 1) ? dA : dBuf, *dst = (it & 1) ? dBuf : dA;
-		if (qbf_inhom_step_periodic(g_eng, src, (int)nv, interm->L, h, dt, dB, dst) ||
-			qbf_d2h(g_eng, &Wevolv[(size_t)it * nv], dst, sz)) {
-			report("SimulationInhomogeneousPeriodic");
-			return;
-		}
-	}
 }
 
 void SimulationInhomogeneousDirichlet(const wignerV_t *restrict_ W0, const quadI1_t *restrict_ quadI1, const quadI2_t *restrict_ quadI2,

@@ -800,6 +800,39 @@ int qbf_set_moment_correction(qbf_engine* e, int mask)
 	return 0;
 }
 
+int qbf_simulate_periodic(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames)
+{
+	if (!e || !dW0 || !hostFrames || ncell < 1 || stride < 1) return set_err(-1, "qbf_simulate_periodic: bad argument");
+	const size_t bytes = sizeof(double) * 4 * (size_t)e->gridN * e->gridN * ncell;
+	double* buf[2] = {nullptr, nullptr};
+	if (hipMalloc(&buf[0], bytes) != hipSuccess || hipMalloc(&buf[1], bytes) != hipSuccess) {
+		if (buf[0]) hipFree(buf[0]);
+		return set_err(-3, "qbf_simulate_periodic: hipMalloc(%zu) failed", bytes);
+	}
+	int r = 0;
+	const double* cur = dW0;
+	size_t f = 0;
+	// frame 0 is the initial state (src/simulation.c:1294)
+	if (hipMemcpyAsync(hostFrames, dW0, bytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess)
+		r = set_err(-2, "qbf_simulate_periodic: copy of frame 0 failed");
+	f = 1;
+	for (unsigned it = 1; it < numsteps && !r; it++) {
+		double* nxt = buf[it & 1];
+		r = e->inhom_step(cur, ncell, L, h, dt, dBext, nxt);
+		cur = nxt;
+		if (!r && it % stride == 0) {
+			if (hipMemcpyAsync((char*)hostFrames + f * bytes, cur, bytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess)
+				r = set_err(-2, "qbf_simulate_periodic: frame copy failed");
+			f++;
+		}
+	}
+	if (hipStreamSynchronize(e->stream) != hipSuccess && !r) r = set_err(-2, "qbf_simulate_periodic: stream error");
+	hipFree(buf[0]);
+	hipFree(buf[1]);
+	return r;
+}
+
 int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, double* dOut)
 {
 	if (!e || !dW || !dOut || ncell < 1) return set_err(-1, "qbf_cell_diagnostics: bad argument");

@@ -80,6 +80,7 @@ def lib():
             "qbf_set_moment_correction": (i, [vp, i]),
             "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
             "qbf_cell_diagnostics": (i, [vp, vp, i, d, vp]),
+            "qbf_simulate_periodic": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
             "qbf_profile": (i, [vp, i]),
             "qbf_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]),
             "qbf_legendre_gauss": (None, [u, d, d, _dp, _dp]),
@@ -178,6 +179,16 @@ class Engine:
         self._bind()
         _check(lib().qbf_moment_correct(self._h, _ptr(Wold), _ptr(Wnew), Wold.shape[0], L, mask))
         return Wnew
+
+    def simulate_periodic(self, W0, L, h, dt, numsteps, stride=1, Bext=None):
+        """Frames 0, stride, 2 stride, ... (< numsteps) of the periodic problem from W0, as a
+        numpy array [nframes, ncell, 4, N, N] (the state stays on the device between steps)."""
+        self._bind()
+        nf = (numsteps - 1) // stride + 1 if numsteps > 0 else 0
+        out = np.empty((max(nf, 1),) + tuple(W0.shape))
+        _check(lib().qbf_simulate_periodic(self._h, _ptr(W0), W0.shape[0], L, h, dt, numsteps, stride, _ptr(Bext),
+                                           out.ctypes.data_as(C.c_void_p)))
+        return out[:nf]
 
     def cell_diagnostics(self, W, L):
         """[ncell, 10] float64 cuda tensor: mass, spin x/y/z, momentum x/y, energy, entropy,
