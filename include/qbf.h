@@ -80,6 +80,14 @@ int qbf_fourier_to_phys(qbf_engine *e, const double *dFourier, int ngrid, double
 /* slope-limited transport, periodic in x, over ncell physical states (src/finite_volume.c:52-104) */
 int qbf_transport_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt, double *dWout);
 
+/* slope-limited transport with Dirichlet ends over ncell >= 2 physical states (src/finite_volume.c:110-170) */
+int qbf_transport_dirichlet(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt, double *dWout);
+
+/* one Strang step with Dirichlet ends: half transport, collision of the interior cells
+ * 1 .. ncell-2 (the end cells keep their transported values), half transport (src/simulation.c:562-830) */
+int qbf_inhom_step_dirichlet(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
+	const double *dBext, double *dWnext);
+
 /* slope-limited transport of one periodic line of n cells with velocity A */
 int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A, double h, double dt, double *dOut);
 

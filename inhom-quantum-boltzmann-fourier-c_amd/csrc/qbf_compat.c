@@ -374,26 +374,17 @@ void InhomStepDirichlet(const wignerV_t *restrict_ W, const quadI1_t *restrict_ 
 	const quadI3_t *restrict_ quadI3, const quadI1_t *restrict_ quadI4,
 	const double h, const double dt, const double *restrict_ Bext, inhomStepInterm_t *restrict_ interm, wignerV_t *restrict_ Wnext)
 {
+	/* half transport with Dirichlet ends, collision on the interior cells only (the end cells
+	 * keep their transported values, src/simulation.c:715), half transport (src/simulation.c:562-830),
+	 * all on the GPU */
 	const unsigned nv = interm->numVol;
-	inhom_impl_t *im = interm->impl;
 	if (!engine((int)nv) || use_all(quadI1, quadI2, quadI3, quadI4)) return;
-	wignerV_t *Wt = fftw_malloc(nv * sizeof(wignerV_t));
-	double *x = malloc(nv * sizeof(double)), *x1 = malloc(nv * sizeof(double));
-	/* half transport, collision on the interior cells only (boundary cells keep their
-	 * transported values, src/simulation.c:715), half transport (src/simulation.c:562-830) */
-	for (int pass = 0; pass < 2; pass++) {
-		const wignerV_t *src = pass == 0 ? W : Wt;
-		wignerV_t *dst = pass == 0 ? Wt : Wnext;
-		for (int j = 0; j < 4; j++)
-			for (int l = 0; l < N_GRID; l++)
-				for (int k = 0; k < N_GRID; k++) {
-					for (unsigned i = 0; i < nv; i++) x[i] = src[i].comp[j].data[k + N_GRID * l];
-					SlopeLimiterStepDirichlet(h, 0.5 * dt, im->vgrid[l], nv, x, x1);
-					for (unsigned i = 0; i < nv; i++) dst[i].comp[j].data[k + N_GRID * l] = x1[i];
-				}
-		if (pass == 0 && nv > 2 && collide_cells(Wt, 1, nv - 1, dt, Bext)) report("InhomStepDirichlet");
-	}
-	free(x1); free(x); fftw_free(Wt);
+	const size_t sz = nv * sizeof(wignerV_t);
+	double *dW = dbuf(0, sz), *dN = dbuf(1, sz), *dB = dbuf(2, 3 * nv * sizeof(double));
+	if (!dW || !dN || !dB) return;
+	if (qbf_h2d(g_eng, dW, W, sz) || qbf_h2d(g_eng, dB, Bext, 3 * nv * sizeof(double)) ||
+		qbf_inhom_step_dirichlet(g_eng, dW, (int)nv, interm->L, h, dt, dB, dN) || qbf_d2h(g_eng, Wnext, dN, sz))
+		report("InhomStepDirichlet");
 }
 
 /* Maxwell velocity grid helpers (src/simulation.c:837-880): drop the zero x-velocity row,

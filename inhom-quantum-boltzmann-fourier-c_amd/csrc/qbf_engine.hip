@@ -107,6 +107,17 @@ struct qbf_engine {
 	virtual int fourier2phys(const double2* in, int ngrid, double* out) = 0;
 	virtual int transport(const double* W, int ncell, double L, double h, double dt, double* out) = 0;
 	virtual int inhom_step(const double* W, int ncell, double L, double h, double dt, const double* Bext, double* out) = 0;
+	virtual int inhom_step_dirichlet(const double* W, int ncell, double L, double h, double dt, const double* Bext,
+		double* out) = 0;
+	int transport_dirichlet(const double* W, int ncell, double L, double h, double dt, double* out)
+	{
+		if (ncell < 2) return set_err(-1, "Dirichlet transport needs at least 2 cells");
+		const int chunk = ncell < 32 ? ncell : 32;
+		const long long nthr = 4LL * gridN * gridN * ((ncell + chunk - 1) / chunk);
+		hipLaunchKernelGGL(k_transport_dirichlet, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, stream, W, out,
+			gridN, ncell, chunk, h, dt, L);
+		return hipGetLastError() == hipSuccess ? 0 : set_err(-2, "k_transport_dirichlet launch failed");
+	}
 	int moment_correct(const double* Wold, double* Wnew, int ncell, double L, int mask)
 	{
 		hipLaunchKernelGGL(k_moment_correct, dim3(ncell), dim3(256), 0, stream, Wold, Wnew, gridN, L, mask);
@@ -601,6 +612,36 @@ struct EngineT final : public qbf_engine {
 		if (!r) r = transport(Wt2.template as<double>(), ncell, L, h, 0.5 * dt, out);
 		return r;
 	}
+
+	// Dirichlet step (src/simulation.c:562-830): half transport with Dirichlet ends, collision on
+	// the interior cells 1 .. ncell-2 only (the end cells keep their transported values), half
+	// transport.  The conservation correction applies to the collided cells.
+	int inhom_step_dirichlet(const double* W, int ncell, double L, double h, double dt, const double* Bext,
+		double* out) override
+	{
+		if (ncell < 2 || ncell > maxCells) return set_err(-1, "ncell %d outside [2, %d]", ncell, maxCells);
+		const size_t cellB = sizeof(double) * 4 * NN;
+		const size_t nphys = cellB * (size_t)ncell;
+		int r = Wt1.ensure(nphys);
+		if (!r) r = Wt2.ensure(nphys);
+		if (!r) r = WF.ensure(2 * nphys);
+		if (!r) r = WF2.ensure(2 * nphys);
+		if (r) return r;
+		double* t1 = Wt1.template as<double>();
+		double* t2 = Wt2.template as<double>();
+		r = transport_dirichlet(W, ncell, L, h, 0.5 * dt, t1);
+		if (!r) HIPCHK(hipMemcpyAsync(t2, t1, nphys, hipMemcpyDeviceToDevice, stream));
+		const int ni = ncell - 2;
+		if (!r && ni > 0) {
+			const size_t off = 4 * NN;   // one cell, in doubles
+			r = phys2fourier(t1 + off, 4 * ni, WF.template as<double2>());
+			if (!r) r = collision_step(WF.template as<double2>(), ni, dt, Bext ? Bext + 3 : nullptr, WF2.template as<double2>());
+			if (!r) r = fourier2phys(WF2.template as<double2>(), 4 * ni, t2 + off);
+			if (!r && momentMask) r = moment_correct(t1 + off, t2 + off, ni, L, momentMask);
+		}
+		if (!r) r = transport_dirichlet(t2, ncell, L, h, 0.5 * dt, out);
+		return r;
+	}
 };
 
 template <typename R, int N>
@@ -791,6 +832,19 @@ int qbf_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L,
 {
 	if (!e || !W || !out) return set_err(-1, "null argument");
 	return e->inhom_step(W, ncell, L, h, dt, Bext, out);
+}
+
+int qbf_inhom_step_dirichlet(qbf_engine* e, const double* W, int ncell, double L, double h, double dt,
+	const double* Bext, double* out)
+{
+	if (!e || !W || !out) return set_err(-1, "null argument");
+	return e->inhom_step_dirichlet(W, ncell, L, h, dt, Bext, out);
+}
+
+int qbf_transport_dirichlet(qbf_engine* e, const double* W, int ncell, double L, double h, double dt, double* out)
+{
+	if (!e || !W || !out) return set_err(-1, "null argument");
+	return e->transport_dirichlet(W, ncell, L, h, dt, out);
 }
 
 int qbf_set_moment_correction(qbf_engine* e, int mask)

@@ -431,6 +431,50 @@ __global__ __launch_bounds__(256) void k_transport_periodic(const double* __rest
 	}
 }
 
+// Slope-limited transport with Dirichlet ends (src/finite_volume.c:110-170) over all lines
+// (component, l, k) of ncell physical states, one thread per line and chunk of cells, the same
+// expression order as the single-line host restatement (contraction off, so bit-identical):
+// h beta_j = minmod(alpha_j, alpha_{j-1}) inside, 0 at both ends; the upwind end cell keeps its
+// value, the downwind end cell takes the one-sided update.
+__global__ __launch_bounds__(256) void k_transport_dirichlet(const double* __restrict__ Un, double* __restrict__ Un1,
+	int N, int ncell, int chunk, double h, double dt, double L)
+{
+#pragma clang fp contract(off)
+	const int NN = N * N;
+	const long long nel = 4LL * NN;
+	const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int nchunks = (ncell + chunk - 1) / chunk;
+	if (gid >= nel * nchunks) return;
+	const int e = (int)(gid % nel);
+	const int ch = (int)(gid / nel);
+	const int l = (e % NN) / N;
+	const double scl = (2 * L) / N;
+	const double A = l < N / 2 ? scl * l : scl * (l - N);
+	const double Aabs = fabs(A);
+	const double nu = dt * A / h;
+	const double S = 0.5 * A * (copysign(1.0, nu) - nu);
+	const double Sabs = 0.5 * Aabs * (copysign(1.0, nu) - nu);
+	const double scale = dt / (2 * h);
+	const int n = ncell;
+	auto U = [&](int c) -> double { return Un[(size_t)c * nel + e]; };
+	auto HB = [&](int j) -> double { return (j <= 0 || j >= n - 1) ? 0.0 : minmod_d(U(j + 1) - U(j), U(j) - U(j - 1)); };
+	const int c0 = ch * chunk, c1 = min(n, c0 + chunk);
+	for (int c = c0; c < c1; c++) {
+		double r;
+		if (c == 0) {
+			r = A >= 0 ? U(0) : U(0) + 2 * scale * (-A * (U(1) - U(0)) - S * (HB(1) - HB(0)));
+		} else if (c == n - 1) {
+			r = A >= 0 ? U(n - 1) + 2 * scale * (-A * (U(n - 1) - U(n - 2)) - S * (HB(n - 1) - HB(n - 2))) : U(n - 1);
+		} else {
+			const double bm = HB(c - 1), b0 = HB(c), bp = HB(c + 1);
+			r = U(c) + scale * (
+				-A * (U(c + 1) - U(c - 1)) + Aabs * (U(c + 1) - 2 * U(c) + U(c - 1))
+				- S * (bp - bm) + Sabs * (bp - 2 * b0 + bm));
+		}
+		Un1[(size_t)c * nel + e] = r;
+	}
+}
+
 // One line of n cells with velocity A (the single-line entry point of the reference,
 // src/finite_volume.c:52-104); same arithmetic as k_transport_periodic.
 __global__ __launch_bounds__(256) void k_transport_line(const double* __restrict__ Un, double* __restrict__ Un1,

@@ -77,6 +77,8 @@ def lib():
             "qbf_fourier_to_phys": (i, [vp, vp, i, vp]),
             "qbf_transport_periodic": (i, [vp, vp, i, d, d, d, vp]),
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
+            "qbf_inhom_step_dirichlet": (i, [vp, vp, i, d, d, d, vp, vp]),
+            "qbf_transport_dirichlet": (i, [vp, vp, i, d, d, d, vp]),
             "qbf_set_moment_correction": (i, [vp, i]),
             "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
             "qbf_cell_diagnostics": (i, [vp, vp, i, d, vp]),
@@ -281,6 +283,18 @@ class Engine:
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
         _check(lib().qbf_transport_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(out)))
+        return out
+
+    def transport_dirichlet(self, W, L, h, dt, out=None):
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_transport_dirichlet(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(out)))
+        return out
+
+    def inhom_step_dirichlet(self, W, L, h, dt, Bext=None, out=None):
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_inhom_step_dirichlet(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
         return out
 
     def inhom_step_periodic(self, W, L, h, dt, Bext=None, out=None):

@@ -57,6 +57,7 @@ def lib():
             "orc_r2c_fill": (None, [i, _dp, _dp]),
             "orc_c2r_real": (None, [i, _dp, _dp]),
             "orc_InhomStepPeriodic": (None, [C.c_void_p, u, d, _dp, d, d, _dp, _dp]),
+            "orc_InhomStepDirichlet": (None, [C.c_void_p, u, d, _dp, d, d, _dp, _dp]),
             "orc_SimulationHomogeneous": (None, [C.c_void_p, _dp, d, u, _dp, _dp]),
             "orc_SlopeLimiterStepPeriodic": (None, [d, d, d, u, _dp, _dp]),
             "orc_SlopeLimiterStepDirichlet": (None, [d, d, d, u, _dp, _dp]),
@@ -256,6 +257,15 @@ def inhom_step_periodic(q: Quad, W, h, dt, Bext=None):
     B = np.zeros(3 * nv) if Bext is None else _c(Bext, np.float64)
     out = np.empty_like(W)
     lib().orc_InhomStepPeriodic(q.ptr, nv, q.L, _p(W), h, dt, _p(B), _p(out))
+    return out
+
+
+def inhom_step_dirichlet(q: Quad, W, h, dt, Bext=None):
+    W = _c(W, np.float64)
+    nv = W.shape[0]
+    B = np.zeros(3 * nv) if Bext is None else _c(Bext, np.float64)
+    out = np.empty_like(W)
+    lib().orc_InhomStepDirichlet(q.ptr, nv, q.L, _p(W), h, dt, _p(B), _p(out))
     return out
 
 

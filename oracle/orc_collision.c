@@ -13,6 +13,7 @@
  *   CollisionStep (midpoint rule) ..... src/simulation.c:105-136
  *   r2c + Hermitian fill, c2c + Re .... src/simulation.c:224-283 (also 1229-1279)
  *   InhomStepPeriodic (non-MPI) ....... src/simulation.c:307-555 (:422-449, 452-453, 524-549)
+ *   InhomStepDirichlet (non-MPI) ...... src/simulation.c:562-830 (collision skips the end cells, :715)
  *   SimulationHomogeneous ............. src/simulation.c:1219-1281
  * The reference's term order, pointwise-product operand order and scale
  * factors are kept, so this computes the reference's own (un-reassociated)
@@ -353,6 +354,42 @@ void orc_InhomStepPeriodic(const orc_quad_t *q, unsigned numVol, double L, const
 	for (int i = 0; i < (int)numVol; i++)
 		fourier_collision_cell(q, &Wtmp[i * cell], dt, &Bext[3 * i], &Wtmp2[i * cell]);
 	half_transport(N, numVol, L, h, dt, Wtmp2, Wnext);
+	free(Wtmp2); free(Wtmp);
+}
+
+static void half_transport_dirichlet(int N, unsigned numVol, double L, double h, double dt, const double *Win,
+	double *Wout)
+{
+	const size_t NN = (size_t)N * N;
+	const double scale = (2 * L) / N;
+	double *Wx = malloc(sizeof(double) * numVol), *Wx1 = malloc(sizeof(double) * numVol);
+	for (int j = 0; j < 4; j++)
+		for (int l = 0; l < N; l++) {
+			const double v = l < N / 2 ? scale * l : scale * (l - N);
+			for (int k = 0; k < N; k++) {
+				const size_t off = j * NN + k + (size_t)N * l;
+				for (unsigned i = 0; i < numVol; i++) Wx[i] = Win[i * 4 * NN + off];
+				orc_SlopeLimiterStepDirichlet(h, 0.5 * dt, v, numVol, Wx, Wx1);
+				for (unsigned i = 0; i < numVol; i++) Wout[i * 4 * NN + off] = Wx1[i];
+			}
+		}
+	free(Wx1); free(Wx);
+}
+
+/* half transport with Dirichlet ends, collision of the cells 1 .. numVol-2 (the end cells keep
+ * their transported values), half transport */
+void orc_InhomStepDirichlet(const orc_quad_t *q, unsigned numVol, double L, const double *W,
+	double h, double dt, const double *Bext, double *Wnext)
+{
+	const int N = q->N;
+	const size_t cell = 4 * (size_t)N * N;
+	double *Wtmp = malloc(sizeof(double) * cell * numVol), *Wtmp2 = malloc(sizeof(double) * cell * numVol);
+	half_transport_dirichlet(N, numVol, L, h, dt, W, Wtmp);
+	memcpy(Wtmp2, Wtmp, sizeof(double) * cell * numVol);
+	#pragma omp parallel for schedule(dynamic, 1)
+	for (int i = 1; i < (int)numVol - 1; i++)
+		fourier_collision_cell(q, &Wtmp[i * cell], dt, &Bext[3 * i], &Wtmp2[i * cell]);
+	half_transport_dirichlet(N, numVol, L, h, dt, Wtmp2, Wnext);
 	free(Wtmp2); free(Wtmp);
 }
 

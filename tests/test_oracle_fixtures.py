@@ -116,3 +116,30 @@ def test_slope_limiters():  # test/slope_lim_test.c and slope_lim_test2.c
     out = O.slope_limiter_maxwell(0.01, 0.005, [1.0, 0.5, -0.5, -1.0], 0.4, 0.7, 0.45,
                                   [0.2, -np.pi / 4, 0, 0], [0, 0, 1 / np.e, -0.8], read_dat("UnM.dat"))
     assert np.array_equal(out, read_dat("Un1_Maxwell_ref.dat"))
+
+
+def test_oracle_inhom_step_dirichlet_structure():
+    """With two cells there is no interior cell: the Dirichlet step is exactly two Dirichlet half
+    transports (src/simulation.c:715 skips the end cells); with three, the collision of the middle
+    cell changes the result."""
+    import oracle as O
+    from conftest import rand_state
+    N, L, h, dt = 8, 6.0, 0.08, 0.002
+    q = O.Quad(N, 4, L, 3.75, 2)
+    v = np.array([k if k < N // 2 else k - N for k in range(N)]) * (2 * L / N)
+
+    def transport(W):
+        out = np.empty_like(W)
+        for c in range(4):
+            for l in range(N):
+                for k in range(N):
+                    out[:, c, l, k] = O.slope_limiter_dirichlet(h, 0.5 * dt, v[l], W[:, c, l, k])
+        return out
+
+    rng = np.random.default_rng(3)
+    W2 = rand_state(rng, 2, N, L)
+    assert np.array_equal(O.inhom_step_dirichlet(q, W2, h, dt), transport(transport(W2)))
+    W3 = rand_state(rng, 3, N, L)
+    got = O.inhom_step_dirichlet(q, W3, h, dt)
+    ref = transport(transport(W3))
+    assert np.abs(got - ref).max() > 0
