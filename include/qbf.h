@@ -88,6 +88,18 @@ int qbf_transport_dirichlet(qbf_engine *e, const double *dW, int ncell, double L
 int qbf_inhom_step_dirichlet(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
 	const double *dBext, double *dWnext);
 
+/* slope-limited transport with Maxwell reflection at both ends (src/finite_volume.c:241-323 on the
+ * Maxwell velocity grid, src/simulation.c:837-1051): the zero x-velocity row is kept, the incoming
+ * ghost states are (1 - lambda) * reflected end cell + lambda * outgoing trace flux * boundary state
+ * normalised by its incoming trace flux.  dWMaxwL, dWMaxwR: one physical state each, standard grid. */
+int qbf_transport_maxwell(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt, double lambda,
+	const double *dWMaxwL, const double *dWMaxwR, double *dWout);
+
+/* one Strang step with Maxwell boundaries: half transport, collision of every cell, half transport
+ * (replaces InhomStepMaxwell's compute, src/simulation.c:882-1165) */
+int qbf_inhom_step_maxwell(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt, double lambda,
+	const double *dWMaxwL, const double *dWMaxwR, const double *dBext, double *dWnext);
+
 /* slope-limited transport of one periodic line of n cells with velocity A */
 int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A, double h, double dt, double *dOut);
 

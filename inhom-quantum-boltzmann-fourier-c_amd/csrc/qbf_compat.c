@@ -317,7 +317,6 @@ void CollisionStep(const wignerF_t *restrict_ W, const quadI1_t *restrict_ quadI
 
 typedef struct {
 	double vgrid[N_GRID];
-	grid_t vxMaxw;
 } inhom_impl_t;
 
 void InhomStepInterm_Create(const unsigned int numVol, const double L, inhomStepInterm_t *interm)
@@ -327,11 +326,6 @@ void InhomStepInterm_Create(const unsigned int numVol, const double L, inhomStep
 	inhom_impl_t *im = calloc(1, sizeof(inhom_impl_t));
 	const double scale = (2 * L) / N_GRID;
 	for (int k = 0; k < N_GRID; k++) im->vgrid[k] = k < N_GRID / 2 ? scale * k : scale * (k - N_GRID);
-	/* Maxwell grid: x velocity depends on the slow index l, zero velocity omitted, +L added
-	 * (src/simulation.c:155-161) */
-	for (int l = 0; l < N_GRID; l++)
-		for (int k = 0; k < N_GRID; k++)
-			im->vxMaxw.data[k + N_GRID * l] = l < N_GRID / 2 ? scale * (l + 1) : scale * (l - N_GRID);
 	interm->impl = im;
 	engine((int)numVol);
 }
@@ -387,61 +381,26 @@ void InhomStepDirichlet(const wignerV_t *restrict_ W, const quadI1_t *restrict_ 
 		report("InhomStepDirichlet");
 }
 
-/* Maxwell velocity grid helpers (src/simulation.c:837-880): drop the zero x-velocity row,
- * duplicate the -L row as +L. */
-static void to_maxwell(const grid_t *c, grid_t *m)
-{
-	memcpy(m->data, &c->data[N_GRID], sizeof(grid_t) / 2);
-	memcpy(&m->data[NN_ / 2], &c->data[NN_ / 2], sizeof(grid_t) / 2);
-}
-static void from_maxwell(const grid_t *m, const double *v0, grid_t *c)
-{
-	memcpy(c->data, v0, N_GRID * sizeof(double));
-	memcpy(&c->data[N_GRID], m->data, sizeof(grid_t) / 2 - N_GRID * sizeof(double));
-	memcpy(&c->data[NN_ / 2], &m->data[NN_ / 2], sizeof(grid_t) / 2);
-}
-
 void InhomStepMaxwell(const wignerV_t *restrict_ W, const quadI1_t *restrict_ quadI1, const quadI2_t *restrict_ quadI2,
 	const quadI3_t *restrict_ quadI3, const quadI1_t *restrict_ quadI4,
 	const double h, const double dt, const double lambda, const wignerV_t *restrict_ WMaxwL, const wignerV_t *restrict_ WMaxwR,
 	const double *restrict_ Bext, inhomStepInterm_t *restrict_ interm, wignerV_t *restrict_ Wnext)
 {
+	/* half transport with Maxwell reflection, collision of every cell, half transport
+	 * (src/simulation.c:882-1165), all on the GPU; the boundary states travel as given and are
+	 * moved to the Maxwell velocity grid and normalised on the device */
 	const unsigned nv = interm->numVol;
-	inhom_impl_t *im = interm->impl;
 	if (!engine((int)nv) || use_all(quadI1, quadI2, quadI3, quadI4)) return;
-	wignerV_t *WL = malloc(sizeof(wignerV_t)), *WR = malloc(sizeof(wignerV_t));
-	for (int j = 0; j < 4; j++) {
-		to_maxwell(&WMaxwL->comp[j], &WL->comp[j]);
-		to_maxwell(&WMaxwR->comp[j], &WR->comp[j]);
-	}
-	/* normalise the incoming boundary states by their trace flux (src/simulation.c:905-918) */
-	const double nL = 1.0 / CalculateRightwardFlux(NN_, im->vxMaxw.data, WL->comp[0].data);
-	const double nR = 1.0 / CalculateLeftwardFlux(NN_, im->vxMaxw.data, WR->comp[0].data);
-	for (int j = 0; j < 4; j++)
-		for (int e = 0; e < NN_; e++) {
-			WL->comp[j].data[e] *= nL;
-			WR->comp[j].data[e] *= nR;
-		}
-	wignerV_t *Wt = fftw_malloc(nv * sizeof(wignerV_t));
-	grid_t *Wx = fftw_malloc(nv * sizeof(grid_t)), *Wx1 = fftw_malloc(nv * sizeof(grid_t));
-	grid_t *trL = malloc(sizeof(grid_t)), *trR = malloc(sizeof(grid_t));
-	for (int pass = 0; pass < 2; pass++) {
-		const wignerV_t *src = pass == 0 ? W : Wt;
-		wignerV_t *dst = pass == 0 ? Wt : Wnext;
-		/* outgoing trace flux at both ends of the current state */
-		to_maxwell(&src[0].comp[0], trL);
-		to_maxwell(&src[nv - 1].comp[0], trR);
-		const double fluxL = CalculateLeftwardFlux(NN_, im->vxMaxw.data, trL->data);
-		const double fluxR = CalculateRightwardFlux(NN_, im->vxMaxw.data, trR->data);
-		for (int j = 0; j < 4; j++) {
-			for (unsigned i = 0; i < nv; i++) to_maxwell(&src[i].comp[j], &Wx[i]);
-			SlopeLimiterStepMaxwell(h, 0.5 * dt, NN_, im->vxMaxw.data, nv, lambda, fluxL, fluxR,
-				WL->comp[j].data, WR->comp[j].data, Wx->data, Wx1->data);
-			for (unsigned i = 0; i < nv; i++) from_maxwell(&Wx1[i], src[i].comp[j].data, &dst[i].comp[j]);
-		}
-		if (pass == 0 && collide_cells(Wt, 0, nv, dt, Bext)) report("InhomStepMaxwell");
-	}
-	free(trR); free(trL); fftw_free(Wx1); fftw_free(Wx); fftw_free(Wt); free(WR); free(WL);
+	const size_t sz = nv * sizeof(wignerV_t);
+	double *dW = dbuf(0, sz), *dN = dbuf(1, sz), *dB = dbuf(2, 3 * nv * sizeof(double));
+	double *dM = dbuf(6, 2 * sizeof(wignerV_t));
+	if (!dW || !dN || !dB || !dM) return;
+	double *dR = dM + sizeof(wignerV_t) / sizeof(double);
+	if (qbf_h2d(g_eng, dW, W, sz) || qbf_h2d(g_eng, dB, Bext, 3 * nv * sizeof(double)) ||
+		qbf_h2d(g_eng, dM, WMaxwL, sizeof(wignerV_t)) || qbf_h2d(g_eng, dR, WMaxwR, sizeof(wignerV_t)) ||
+		qbf_inhom_step_maxwell(g_eng, dW, (int)nv, interm->L, h, dt, lambda, dM, dR, dB, dN) ||
+		qbf_d2h(g_eng, Wnext, dN, sz))
+		report("InhomStepMaxwell");
 }
 
 void SimulationHomInterm_Create(simulationHomInterm_t *interm) { interm->impl = NULL; }
@@ -512,8 +471,8 @@ void SimulationInhomogeneousMaxwell(const wignerV_t *restrict_ W0, const quadI1_
 
 /* ------------------------------------------------------------------ finite_volume.h
  * Single-line host entry points (src/finite_volume.c).  The periodic one runs on the GPU
- * through the batched transport kernel; the Dirichlet and Maxwell boundary variants are
- * host code (they serve boundary conditions outside the accelerated configurations). */
+ * through the batched transport kernel; the single-line Dirichlet and Maxwell variants are host
+ * utilities (the Dirichlet and Maxwell steps above run their batched kernels on the GPU). */
 static double minmod(double a, double b)
 {
 	return 0.5 * (copysign(1, a) + copysign(1, b)) * fmin(fabs(a), fabs(b));

@@ -68,6 +68,7 @@ template <> struct CfgNN<64> { static constexpr int T = 256; };
 struct qbf_engine {
 	virtual ~qbf_engine()
 	{
+		if (sflux) (void)hipFree(sflux);
 		for (auto& ev : evs) {
 			hipEventDestroy(ev.first);
 			hipEventDestroy(ev.second);
@@ -117,6 +118,27 @@ struct qbf_engine {
 		hipLaunchKernelGGL(k_transport_dirichlet, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, stream, W, out,
 			gridN, ncell, chunk, h, dt, L);
 		return hipGetLastError() == hipSuccess ? 0 : set_err(-2, "k_transport_dirichlet launch failed");
+	}
+	// Maxwell boundaries: trace fluxes of the current state into sflux, then the limiter with the
+	// reflected ghost cells (WL, WR: the boundary states on the standard grid, not normalised)
+	double* sflux = nullptr;
+	virtual int inhom_step_maxwell(const double* W, int ncell, double L, double h, double dt, double lambda,
+		const double* WL, const double* WR, const double* Bext, double* out) = 0;
+	int transport_maxwell(const double* W, int ncell, double L, double h, double dt, double lambda,
+		const double* WL, const double* WR, double* out)
+	{
+		if (ncell < 1) return set_err(-1, "Maxwell transport needs at least 1 cell");
+		if (!sflux && hipMalloc(&sflux, 4 * sizeof(double)) != hipSuccess) {
+			sflux = nullptr;
+			return set_err(-3, "hipMalloc of the trace fluxes failed");
+		}
+		hipLaunchKernelGGL(k_maxwell_flux, dim3(1), dim3(256), 0, stream, W, gridN, ncell, L, WL, WR, sflux);
+		if (hipGetLastError() != hipSuccess) return set_err(-2, "k_maxwell_flux launch failed");
+		const int chunk = ncell < 32 ? ncell : 32;
+		const long long nthr = 4LL * gridN * gridN * ((ncell + chunk - 1) / chunk);
+		hipLaunchKernelGGL(k_transport_maxwell, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, stream, W, out,
+			gridN, ncell, chunk, h, dt, L, lambda, WL, WR, (const double*)sflux);
+		return hipGetLastError() == hipSuccess ? 0 : set_err(-2, "k_transport_maxwell launch failed");
 	}
 	int moment_correct(const double* Wold, double* Wnew, int ncell, double L, int mask)
 	{
@@ -642,6 +664,29 @@ struct EngineT final : public qbf_engine {
 		if (!r) r = transport_dirichlet(t2, ncell, L, h, 0.5 * dt, out);
 		return r;
 	}
+
+	// Maxwell step (src/simulation.c:882-1165): half transport with Maxwell reflection, collision of
+	// every cell, half transport; the trace fluxes are taken from the state each half transport starts from.
+	int inhom_step_maxwell(const double* W, int ncell, double L, double h, double dt, double lambda,
+		const double* WL, const double* WR, const double* Bext, double* out) override
+	{
+		if (ncell < 1 || ncell > maxCells) return set_err(-1, "ncell %d outside [1, %d]", ncell, maxCells);
+		const size_t nphys = sizeof(double) * 4 * NN * (size_t)ncell;
+		int r = Wt1.ensure(nphys);
+		if (!r) r = Wt2.ensure(nphys);
+		if (!r) r = WF.ensure(2 * nphys);
+		if (!r) r = WF2.ensure(2 * nphys);
+		if (r) return r;
+		double* t1 = Wt1.template as<double>();
+		double* t2 = Wt2.template as<double>();
+		r = transport_maxwell(W, ncell, L, h, 0.5 * dt, lambda, WL, WR, t1);
+		if (!r) r = phys2fourier(t1, 4 * ncell, WF.template as<double2>());
+		if (!r) r = collision_step(WF.template as<double2>(), ncell, dt, Bext, WF2.template as<double2>());
+		if (!r) r = fourier2phys(WF2.template as<double2>(), 4 * ncell, t2);
+		if (!r && momentMask) r = moment_correct(t1, t2, ncell, L, momentMask);
+		if (!r) r = transport_maxwell(t2, ncell, L, h, 0.5 * dt, lambda, WL, WR, out);
+		return r;
+	}
 };
 
 template <typename R, int N>
@@ -845,6 +890,20 @@ int qbf_transport_dirichlet(qbf_engine* e, const double* W, int ncell, double L,
 {
 	if (!e || !W || !out) return set_err(-1, "null argument");
 	return e->transport_dirichlet(W, ncell, L, h, dt, out);
+}
+
+int qbf_inhom_step_maxwell(qbf_engine* e, const double* W, int ncell, double L, double h, double dt, double lambda,
+	const double* WMaxwL, const double* WMaxwR, const double* Bext, double* out)
+{
+	if (!e || !W || !WMaxwL || !WMaxwR || !out) return set_err(-1, "qbf_inhom_step_maxwell: null argument");
+	return e->inhom_step_maxwell(W, ncell, L, h, dt, lambda, WMaxwL, WMaxwR, Bext, out);
+}
+
+int qbf_transport_maxwell(qbf_engine* e, const double* W, int ncell, double L, double h, double dt, double lambda,
+	const double* WMaxwL, const double* WMaxwR, double* out)
+{
+	if (!e || !W || !WMaxwL || !WMaxwR || !out) return set_err(-1, "qbf_transport_maxwell: null argument");
+	return e->transport_maxwell(W, ncell, L, h, dt, lambda, WMaxwL, WMaxwR, out);
 }
 
 int qbf_set_moment_correction(qbf_engine* e, int mask)

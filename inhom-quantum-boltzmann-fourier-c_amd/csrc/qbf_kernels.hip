@@ -475,6 +475,100 @@ __global__ __launch_bounds__(256) void k_transport_dirichlet(const double* __res
 	}
 }
 
+// Trace fluxes of the Maxwell step on the Maxwell velocity grid (src/finite_volume.c:186-214,
+// src/simulation.c:905-918, 1053-1059): s[0] = leftward flux of the trace of cell 0, s[1] =
+// rightward flux of the trace of cell ncell-1, s[2] = 1 / rightward flux of WL's trace, s[3] =
+// 1 / leftward flux of WR's trace.  The Maxwell grid drops the zero x-velocity row and holds the
+// -L row twice: Maxwell index i < NN/2 is standard index i + N with velocity (l+1)*2L/N, i >= NN/2
+// is standard index i with velocity (l-N)*2L/N (l = i / N).  The products are formed by all
+// threads; each sum then runs in index order in one lane, so the rounding is the reference's.
+__global__ __launch_bounds__(256) void k_maxwell_flux(const double* __restrict__ W, int N, int ncell, double L,
+	const double* __restrict__ WL, const double* __restrict__ WR, double* __restrict__ s)
+{
+#pragma clang fp contract(off)
+	constexpr int TILE = 1024;
+	__shared__ double prod[4][TILE];
+	const int NN = N * N, half = NN / 2;
+	const double scl = (2 * L) / N;
+	const double* Wlast = W + (size_t)(ncell - 1) * 4 * NN;
+	double acc = 0.0;
+	for (int base = 0; base < half; base += TILE) {
+		for (int t = threadIdx.x; t < 4 * TILE; t += blockDim.x) {
+			const int w = t / TILE, i = base + t % TILE;
+			if (i >= half) continue;
+			const bool right = w == 1 || w == 2;                  // rightward sums run over Maxwell i < NN/2
+			const int l = (right ? i : half + i) / N;
+			const double A = l < N / 2 ? scl * (l + 1) : scl * (l - N);
+			const double* tr = w == 0 ? W : w == 1 ? Wlast : w == 2 ? WL : WR;
+			const double u = tr[right ? i + N : half + i];
+			prod[w][t % TILE] = right ? A * u : fabs(A) * u;
+		}
+		__syncthreads();
+		if (threadIdx.x < 4) {
+			const int cnt = min(TILE, half - base);
+			for (int i = 0; i < cnt; i++) acc += prod[threadIdx.x][i];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x < 4) s[threadIdx.x] = threadIdx.x >= 2 ? 1.0 / acc : acc;
+}
+
+// Half transport of ncell cells with Maxwell reflection at both ends (src/finite_volume.c:241-323
+// on the Maxwell grid, src/simulation.c:1020-1029).  Thread = (component, velocity), one chunk of
+// cells.  Standard row r >= 1 is Maxwell row r-1 (r < N/2) or r (r >= N/2) with velocity
+// vgrid[r]; the zero x-velocity row r = 0 is not transported.  The ghost cells V(-2) = V(-1) and
+// V(n) = V(n+1) repeat the end cell on the outgoing side; on the incoming side they mix the
+// reflected end cell (Maxwell index NN-1-i, standard row N-r, column N-1-k) with the normalised
+// boundary state: (1 - lambda) U_refl + (lambda flux) (W_b * norm).  The Maxwell +L copy of the
+// -L row is only read (as a reflection), never written, as in the reference.
+__global__ __launch_bounds__(256) void k_transport_maxwell(const double* __restrict__ Un, double* __restrict__ Un1,
+	int N, int ncell, int chunk, double h, double dt, double L, double lambda, const double* __restrict__ WL,
+	const double* __restrict__ WR, const double* __restrict__ s)
+{
+#pragma clang fp contract(off)
+	const int NN = N * N;
+	const long long nel = 4LL * NN;
+	const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	const int nchunks = (ncell + chunk - 1) / chunk;
+	if (gid >= nel * nchunks) return;
+	const int e = (int)(gid % nel);
+	const int ch = (int)(gid / nel);
+	const int j = e / NN, q = e % NN, r = q / N, k = q % N;
+	const int n = ncell;
+	const int c0 = ch * chunk, c1 = min(n, c0 + chunk);
+	if (r == 0) {
+		for (int c = c0; c < c1; c++) Un1[(size_t)c * nel + e] = Un[(size_t)c * nel + e];
+		return;
+	}
+	const double scl = (2 * L) / N;
+	const double A = r < N / 2 ? scl * r : scl * (r - N);
+	const double Aabs = fabs(A);
+	const double nu = dt * A / h;
+	const double S = 0.5 * A * (copysign(1.0, nu) - nu);
+	const double Sabs = 0.5 * Aabs * (copysign(1.0, nu) - nu);
+	const double scale = dt / (2 * h);
+	const int qr = j * NN + (N - r) * N + (N - 1 - k);
+	double gm, gp;
+	if (r < N / 2) {
+		const double ub = WL[e] * s[2];
+		gm = (1 - lambda) * Un[qr] + lambda * s[0] * ub;
+		gp = Un[(size_t)(n - 1) * nel + e];
+	} else {
+		const double ub = WR[e] * s[3];
+		gm = Un[e];
+		gp = (1 - lambda) * Un[(size_t)(n - 1) * nel + qr] + lambda * s[1] * ub;
+	}
+	auto V = [&](int c) -> double { return c < 0 ? gm : c >= n ? gp : Un[(size_t)c * nel + e]; };
+	auto HB = [&](int c) -> double { return minmod_d(V(c + 1) - V(c), V(c) - V(c - 1)); };
+	for (int c = c0; c < c1; c++) {
+		const double vm = V(c - 1), v0 = V(c), vp = V(c + 1);
+		const double bm = HB(c - 1), b0 = HB(c), bp = HB(c + 1);
+		Un1[(size_t)c * nel + e] = v0 + scale * (
+			-A * (vp - vm) + Aabs * (vp - 2 * v0 + vm)
+			- S * (bp - bm) + Sabs * (bp - 2 * b0 + bm));
+	}
+}
+
 // One line of n cells with velocity A (the single-line entry point of the reference,
 // src/finite_volume.c:52-104); same arithmetic as k_transport_periodic.
 __global__ __launch_bounds__(256) void k_transport_line(const double* __restrict__ Un, double* __restrict__ Un1,

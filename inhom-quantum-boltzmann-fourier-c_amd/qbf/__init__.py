@@ -79,6 +79,8 @@ def lib():
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
             "qbf_inhom_step_dirichlet": (i, [vp, vp, i, d, d, d, vp, vp]),
             "qbf_transport_dirichlet": (i, [vp, vp, i, d, d, d, vp]),
+            "qbf_transport_maxwell": (i, [vp, vp, i, d, d, d, d, vp, vp, vp]),
+            "qbf_inhom_step_maxwell": (i, [vp, vp, i, d, d, d, d, vp, vp, vp, vp]),
             "qbf_set_moment_correction": (i, [vp, i]),
             "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
             "qbf_cell_diagnostics": (i, [vp, vp, i, d, vp]),
@@ -295,6 +297,20 @@ class Engine:
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
         _check(lib().qbf_inhom_step_dirichlet(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
+        return out
+
+    def transport_maxwell(self, W, L, h, dt, lam, WL, WR, out=None):
+        """Slope-limited transport with Maxwell reflection; WL, WR: boundary states [4, N, N] on the device."""
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_transport_maxwell(self._h, _ptr(W), W.shape[0], L, h, dt, lam, _ptr(WL), _ptr(WR), _ptr(out)))
+        return out
+
+    def inhom_step_maxwell(self, W, L, h, dt, lam, WL, WR, Bext=None, out=None):
+        self._bind()
+        out = self._torch.empty_like(W) if out is None else out
+        _check(lib().qbf_inhom_step_maxwell(self._h, _ptr(W), W.shape[0], L, h, dt, lam, _ptr(WL), _ptr(WR),
+                                            _ptr(Bext), _ptr(out)))
         return out
 
     def inhom_step_periodic(self, W, L, h, dt, Bext=None, out=None):

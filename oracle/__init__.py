@@ -58,6 +58,7 @@ def lib():
             "orc_c2r_real": (None, [i, _dp, _dp]),
             "orc_InhomStepPeriodic": (None, [C.c_void_p, u, d, _dp, d, d, _dp, _dp]),
             "orc_InhomStepDirichlet": (None, [C.c_void_p, u, d, _dp, d, d, _dp, _dp]),
+            "orc_InhomStepMaxwell": (None, [C.c_void_p, u, d, _dp, d, d, d, _dp, _dp, _dp, _dp]),
             "orc_SimulationHomogeneous": (None, [C.c_void_p, _dp, d, u, _dp, _dp]),
             "orc_SlopeLimiterStepPeriodic": (None, [d, d, d, u, _dp, _dp]),
             "orc_SlopeLimiterStepDirichlet": (None, [d, d, d, u, _dp, _dp]),
@@ -267,6 +268,26 @@ def inhom_step_dirichlet(q: Quad, W, h, dt, Bext=None):
     out = np.empty_like(W)
     lib().orc_InhomStepDirichlet(q.ptr, nv, q.L, _p(W), h, dt, _p(B), _p(out))
     return out
+
+
+def inhom_step_maxwell(q: Quad, W, h, dt, lam, WL, WR, Bext=None):
+    W = _c(W, np.float64)
+    nv = W.shape[0]
+    B = np.zeros(3 * nv) if Bext is None else _c(Bext, np.float64)
+    out = np.empty_like(W)
+    lib().orc_InhomStepMaxwell(q.ptr, nv, q.L, _p(W), h, dt, lam, _p(_c(WL, np.float64)), _p(_c(WR, np.float64)),
+                               _p(B), _p(out))
+    return out
+
+
+def rightward_flux(A, U):
+    A = _c(A, np.float64)
+    return lib().orc_CalculateRightwardFlux(A.size, _p(A), _p(_c(U, np.float64)))
+
+
+def leftward_flux(A, U):
+    A = _c(A, np.float64)
+    return lib().orc_CalculateLeftwardFlux(A.size, _p(A), _p(_c(U, np.float64)))
 
 
 def simulation_homogeneous(q: Quad, W0, dt, numsteps, Bext=(0.0, 0.0, 0.0)):

@@ -14,6 +14,7 @@
  *   r2c + Hermitian fill, c2c + Re .... src/simulation.c:224-283 (also 1229-1279)
  *   InhomStepPeriodic (non-MPI) ....... src/simulation.c:307-555 (:422-449, 452-453, 524-549)
  *   InhomStepDirichlet (non-MPI) ...... src/simulation.c:562-830 (collision skips the end cells, :715)
+ *   InhomStepMaxwell (non-MPI) ........ src/simulation.c:837-1165 (Maxwell velocity grid, trace fluxes)
  *   SimulationHomogeneous ............. src/simulation.c:1219-1281
  * The reference's term order, pointwise-product operand order and scale
  * factors are kept, so this computes the reference's own (un-reassociated)
@@ -391,6 +392,76 @@ void orc_InhomStepDirichlet(const orc_quad_t *q, unsigned numVol, double L, cons
 		fourier_collision_cell(q, &Wtmp[i * cell], dt, &Bext[3 * i], &Wtmp2[i * cell]);
 	half_transport_dirichlet(N, numVol, L, h, dt, Wtmp2, Wnext);
 	free(Wtmp2); free(Wtmp);
+}
+
+/* Maxwell velocity grid (src/simulation.c:155-161, :837-856): the zero x-velocity row is left
+ * out and the -L row appears twice, as -L (row N/2) and +L (row N/2-1); x velocity on the slow index */
+static void to_maxwell_grid(int N, const double *c, double *m)
+{
+	const size_t half = (size_t)N * N / 2;
+	memcpy(m, &c[N], sizeof(double) * half);
+	memcpy(&m[half], &c[half], sizeof(double) * half);
+}
+
+static void from_maxwell_grid(int N, const double *m, const double *v0, double *c)
+{
+	const size_t half = (size_t)N * N / 2;
+	memcpy(c, v0, sizeof(double) * N);
+	memcpy(&c[N], m, sizeof(double) * (half - N));
+	memcpy(&c[half], &m[half], sizeof(double) * half);
+}
+
+/* half transport with Maxwell reflection at both ends (src/simulation.c:905-1051): outgoing trace
+ * fluxes of the end cells, the limiter on the Maxwell grid per component, zero x-velocity row kept */
+static void half_transport_maxwell(int N, unsigned numVol, const double *vxM, double h, double dt, double lambda,
+	const double *WLeff, const double *WReff, const double *Win, double *Wout)
+{
+	const size_t NN = (size_t)N * N, cell = 4 * NN;
+	double *tr = malloc(sizeof(double) * NN);
+	to_maxwell_grid(N, &Win[0], tr);
+	const double fluxL = orc_CalculateLeftwardFlux((unsigned)NN, vxM, tr);
+	to_maxwell_grid(N, &Win[(numVol - 1) * cell], tr);
+	const double fluxR = orc_CalculateRightwardFlux((unsigned)NN, vxM, tr);
+	double *Wx = malloc(sizeof(double) * NN * numVol), *Wx1 = malloc(sizeof(double) * NN * numVol);
+	for (int j = 0; j < 4; j++) {
+		for (unsigned i = 0; i < numVol; i++) to_maxwell_grid(N, &Win[i * cell + j * NN], &Wx[i * NN]);
+		orc_SlopeLimiterStepMaxwell(h, 0.5 * dt, (unsigned)NN, vxM, numVol, lambda, fluxL, fluxR,
+			&WLeff[j * NN], &WReff[j * NN], Wx, Wx1);
+		for (unsigned i = 0; i < numVol; i++)
+			from_maxwell_grid(N, &Wx1[i * NN], &Win[i * cell + j * NN], &Wout[i * cell + j * NN]);
+	}
+	free(Wx1); free(Wx); free(tr);
+}
+
+/* Maxwell boundary step (src/simulation.c:882-1165): boundary states on the Maxwell grid,
+ * normalised by their incoming trace flux; half transport, collision of every cell, half transport */
+void orc_InhomStepMaxwell(const orc_quad_t *q, unsigned numVol, double L, const double *W, double h, double dt,
+	double lambda, const double *WMaxwL, const double *WMaxwR, const double *Bext, double *Wnext)
+{
+	const int N = q->N;
+	const size_t NN = (size_t)N * N, cell = 4 * NN;
+	const double scale = (2 * L) / N;
+	double *vxM = malloc(sizeof(double) * NN);
+	for (int l = 0; l < N; l++)
+		for (int k = 0; k < N; k++) vxM[k + (size_t)N * l] = l < N / 2 ? scale * (l + 1) : scale * (l - N);
+	double *WL = malloc(sizeof(double) * cell), *WR = malloc(sizeof(double) * cell);
+	for (int j = 0; j < 4; j++) {
+		to_maxwell_grid(N, &WMaxwL[j * NN], &WL[j * NN]);
+		to_maxwell_grid(N, &WMaxwR[j * NN], &WR[j * NN]);
+	}
+	const double normL = 1.0 / orc_CalculateRightwardFlux((unsigned)NN, vxM, WL);
+	const double normR = 1.0 / orc_CalculateLeftwardFlux((unsigned)NN, vxM, WR);
+	for (size_t i = 0; i < cell; i++) {
+		WL[i] *= normL;
+		WR[i] *= normR;
+	}
+	double *Wtmp = malloc(sizeof(double) * cell * numVol), *Wtmp2 = malloc(sizeof(double) * cell * numVol);
+	half_transport_maxwell(N, numVol, vxM, h, dt, lambda, WL, WR, W, Wtmp);
+	#pragma omp parallel for schedule(dynamic, 1)
+	for (int i = 0; i < (int)numVol; i++)
+		fourier_collision_cell(q, &Wtmp[i * cell], dt, &Bext[3 * i], &Wtmp2[i * cell]);
+	half_transport_maxwell(N, numVol, vxM, h, dt, lambda, WL, WR, Wtmp2, Wnext);
+	free(Wtmp2); free(Wtmp); free(WR); free(WL); free(vxM);
 }
 
 void orc_SimulationHomogeneous(const orc_quad_t *q, const double *W0, double dt, unsigned numsteps,

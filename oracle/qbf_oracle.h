@@ -82,6 +82,8 @@ void orc_InhomStepPeriodic(const orc_quad_t *q, unsigned numVol, double L, const
 	double h, double dt, const double *Bext, double *Wnext);
 void orc_InhomStepDirichlet(const orc_quad_t *q, unsigned numVol, double L, const double *W,
 	double h, double dt, const double *Bext, double *Wnext);
+void orc_InhomStepMaxwell(const orc_quad_t *q, unsigned numVol, double L, const double *W, double h, double dt,
+	double lambda, const double *WMaxwL, const double *WMaxwR, const double *Bext, double *Wnext);
 void orc_SimulationHomogeneous(const orc_quad_t *q, const double *W0, double dt, unsigned numsteps,
 	const double Bext[3], double *Wevolv);
 

@@ -43,3 +43,45 @@ def rand_state(rng, ncell, N, L=6.0):
         for i in range(3):
             out[c, i + 1] = 0.8 * w0 * n[i]
     return out
+
+
+def maxwell_vx(N, L):
+    """x velocities of the Maxwell grid, [N*N] with x on the slow index (src/simulation.c:155-161)."""
+    sc = 2 * L / N
+    row = np.array([sc * (l + 1) if l < N // 2 else sc * (l - N) for l in range(N)])
+    return np.repeat(row, N)
+
+
+def maxwell_transport(W, L, h, dt, lam, WL, WR):
+    """Half transport with Maxwell boundaries assembled from the oracle's limiter and trace fluxes
+    (src/simulation.c:882-1051).  The Maxwell grid drops the zero x-velocity row and repeats the -L
+    row as +L; the boundary states are normalised by their incoming trace flux and the ghost cells
+    take the outgoing trace fluxes of the end cells.  `dt` is the full step (the limiter runs dt/2)."""
+    import oracle as O
+    nv, _, N, _ = W.shape
+    A = maxwell_vx(N, L)
+    half = N * N // 2
+
+    def to_m(c):
+        c = c.reshape(-1)
+        return np.concatenate([c[N:N + half], c[half:]])
+
+    def from_m(m, c):
+        out = c.reshape(-1).copy()
+        out[N:half] = m[:half - N]
+        out[half:] = m[half:]
+        return out.reshape(N, N)
+
+    WLm = np.stack([to_m(WL[j]) for j in range(4)])
+    WRm = np.stack([to_m(WR[j]) for j in range(4)])
+    WLm = WLm * (1.0 / O.rightward_flux(A, WLm[0]))
+    WRm = WRm * (1.0 / O.leftward_flux(A, WRm[0]))
+    fL = O.leftward_flux(A, to_m(W[0, 0]))
+    fR = O.rightward_flux(A, to_m(W[-1, 0]))
+    out = np.empty_like(W)
+    for j in range(4):
+        Wx = np.stack([to_m(W[i, j]) for i in range(nv)])
+        Wx1 = O.slope_limiter_maxwell(h, 0.5 * dt, A, lam, fL, fR, WLm[j], WRm[j], Wx).reshape(nv, -1)
+        for i in range(nv):
+            out[i, j] = from_m(Wx1[i], W[i, j])
+    return out

@@ -84,3 +84,34 @@ def test_main_inhom_trajectory(layout):
     out = np.fromfile(layout / "data" / "Wevolv_inhom_periodic.dat").reshape(5, 10, 4, N, N)
     ref = read_dat("Wevolv_inhom_periodic.dat", shape=(5, 10, 4, N, N))
     assert max(l1rel(out[i], ref[i]) for i in range(1, 5)) < 1e-11
+
+
+@pytest.mark.parametrize("btype", ["Dirichlet", "Maxwell"])
+def test_main_inhom_boundaries(layout, btype):
+    """main_inhom with Dirichlet and Maxwell boundaries (src/main_inhom.c:367-380), its parameter
+    file written here, against the oracle's steps; J = M = 4 keeps the oracle at a few seconds."""
+    import oracle as O
+    from conftest import rand_state
+    L, R, h, dt, nv, steps, lam = 12, 7.5, 0.1, 0.008, 10, 3, 0.4
+    lines = ["J = 4", "M = 4", f"L = {L}", f"R = {R}", f"h = {h}", f"numVol = {nv}", f"dt = {dt}",
+             f"numsteps = {steps}", f"boundaryType = {btype}", "filenameWinit = ../data/W0_inhom.dat",
+             f"filenameWevolv = ../data/Wevolv_{btype}.dat"]
+    WL, WR = rand_state(np.random.default_rng(5), 2, N, float(L))
+    if btype == "Maxwell":
+        WL.tofile(layout / "data" / "WMaxwL.dat")
+        WR.tofile(layout / "data" / "WMaxwR.dat")
+        lines += [f"lambda = {lam}", "filenameWMaxwL = ../data/WMaxwL.dat", "filenameWMaxwR = ../data/WMaxwR.dat"]
+    (layout / "bin" / f"params_{btype}.txt").write_text("\n".join(lines) + "\n")
+    run(layout, "main_inhom", f"params_{btype}.txt")
+    out = np.fromfile(layout / "data" / f"Wevolv_{btype}.dat").reshape(steps, nv, 4, N, N)
+    W = read_dat("W0_inhom.dat", shape=(nv, 4, N, N))
+    assert np.array_equal(out[0], W)
+    q = O.Quad(N, 4, float(L), R, 4)
+    for it in range(1, steps):
+        if btype == "Dirichlet":
+            W = O.inhom_step_dirichlet(q, W, h, dt)
+        else:
+            W = O.inhom_step_maxwell(q, W, h, dt, lam, WL, WR)
+        err = l1rel(out[it], W)
+        print(f"main_inhom {btype} step {it}: {err:.2e}")
+        assert err < 1e-11

@@ -143,3 +143,41 @@ def test_oracle_inhom_step_dirichlet_structure():
     got = O.inhom_step_dirichlet(q, W3, h, dt)
     ref = transport(transport(W3))
     assert np.abs(got - ref).max() > 0
+
+
+def test_oracle_trace_fluxes_closed_form():
+    """For U = 1 the rightward flux is N * sum_{l<N/2} (l+1) * 2L/N and the leftward one
+    N * sum_{l>=N/2} (N-l) * 2L/N; both sums run in index order, one rounding per product and
+    per addition (src/finite_volume.c:186-214)."""
+    from conftest import maxwell_vx
+    N, L = 8, 6.0
+    A = maxwell_vx(N, L)
+    sc = 2 * L / N
+    assert O.rightward_flux(A, np.ones(N * N)) == pytest.approx(N * sc * sum(l + 1 for l in range(N // 2)), rel=1e-15)
+    assert O.leftward_flux(A, np.ones(N * N)) == pytest.approx(N * sc * sum(N - l for l in range(N // 2, N)), rel=1e-15)
+    rng = np.random.default_rng(4)
+    U = rng.uniform(0, 1, N * N)
+    acc_r = acc_l = 0.0
+    for i in range(N * N // 2):
+        acc_r += float(A[i] * U[i])
+    for i in range(N * N // 2, N * N):
+        acc_l += float(abs(A[i]) * U[i])
+    assert O.rightward_flux(A, U) == acc_r and O.leftward_flux(A, U) == acc_l
+
+
+def test_oracle_inhom_step_maxwell_composition():
+    """The Maxwell step against its assembly from the limiter (pinned by Un1_Maxwell_ref.dat), the
+    trace fluxes and the single-cell collision (pinned by the collision fixtures): transport,
+    collision of every cell, transport (src/simulation.c:837-1165)."""
+    from conftest import maxwell_transport, rand_state
+    N, L, h, dt, lam, nv = 8, 6.0, 0.08, 0.004, 0.35, 4
+    q = O.Quad(N, 4, L, 3.75, 2)
+    rng = np.random.default_rng(6)
+    W = rand_state(rng, nv, N, L)
+    WL, WR = rand_state(rng, 2, N, L)
+    Wt = maxwell_transport(W, L, h, dt, lam, WL, WR)
+    Wc = np.stack([O.simulation_homogeneous(q, Wt[i], dt, 2)[1] for i in range(nv)])
+    got = O.inhom_step_maxwell(q, W, h, dt, lam, WL, WR)
+    assert np.array_equal(got, maxwell_transport(Wc, L, h, dt, lam, WL, WR))
+    # the zero x-velocity row is not transported: it changes by the collision only
+    assert np.array_equal(got[:, :, 0, :], Wc[:, :, 0, :])
