@@ -156,6 +156,9 @@ def main():
             cur = 1 - cur
             if i in keep_at:
                 kept[i] = bufs[cur].clone()
+            if k >= 50 and i % 25 == 0:          # long runs report their progress (stderr)
+                torch.cuda.synchronize()
+                print(f"  {i}/{k} steps", file=sys.stderr, flush=True)
         return bufs[cur], kept
 
     def moment_errors(W, step_no):
@@ -167,6 +170,7 @@ def main():
         return {"step": step_no,
                 "density_max_rel": float(np.abs(m[:, 0] - mr[:, 0]).max() / np.abs(mr[:, 0]).max()),
                 "energy_max_rel": float(np.abs(m[:, 6] - mr[:, 6]).max() / np.abs(mr[:, 6]).max()),
+                "momentum_max_rel": float(np.abs(m[:, 4:6] - mr[:, 4:6]).max() / np.abs(mr[:, 4:6]).max()),
                 "density_l1_rel": float(np.abs(m[:, 0] - mr[:, 0]).sum() / np.abs(mr[:, 0]).sum()),
                 "energy_l1_rel": float(np.abs(m[:, 6] - mr[:, 6]).sum() / np.abs(mr[:, 6]).sum())}
 
@@ -182,13 +186,15 @@ def main():
         torch.cuda.synchronize()
         e32.profile(True)
         t0 = time.perf_counter()
-        run_steps(step32, W32_1, args.mixed_steps)
+        W32_n, _ = run_steps(step32, W32_1, args.mixed_steps)
         torch.cuda.synchronize()
         tm = time.perf_counter() - t0
         kms, nl, kapps = e32.profile_read()
         e32.profile(False)
         out["mixed"] = {"value": evals_per_step * args.mixed_steps / tm, "ms_per_step": 1e3 * tm / args.mixed_steps,
-                        "steps": args.mixed_steps, "i3": (kms, nl, kapps, tm), "W1": W32_1}
+                        "steps": args.mixed_steps, "i3": (kms, nl, kapps, tm), "W1": W32_1,
+                        "error_vs_fp64": moment_errors(W32_n, 1 + args.mixed_steps)}
+        del W32_n
 
     # ---------------------------------------------------------------- FP64 plain path
     if "fp64" in paths:
@@ -309,6 +315,8 @@ def main():
         mx = out["mixed"]
         line["mixed_fp32"] = {"value": round(mx["value"], 3), "ms_per_step": round(mx["ms_per_step"], 3),
                               "steps": mx["steps"]}
+        if mx["error_vs_fp64"]:
+            line["mixed_fp32"]["error_vs_fp64"] = mx["error_vs_fp64"]
         if "fp64" in out:
             a, b = mx["W1"].cpu().numpy(), out["fp64"]["W1"].cpu().numpy()
             ma, mb = wl.moments(a), wl.moments(b)

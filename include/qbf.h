@@ -131,6 +131,23 @@ int qbf_moment_correct(qbf_engine *e, const double *dWold, double *dWnew, int nc
  * (visualization/wignerMoments.m:15-37, entropy.m, visualizeHomBoltzmann.m:26-45,98-105). */
 int qbf_cell_diagnostics(qbf_engine *e, const double *dW, int ncell, double L, double *dOut);
 
+/* Fermi-Dirac equilibrium with each cell's conserved moments (visualization/asymptoticFD.m,
+ * calculateFDparams.m, createFD.m).  dParams[ncell][12] = beta, mu1, mu2, vel_x, vel_y, n_x, n_y, n_z
+ * (eigenvector of the larger eigenvalue of the spin density matrix), lambda1 <= lambda2 (its
+ * eigenvalues), intrinsic energy per particle, status (0 fitted, 1 no Fermi-Dirac state has these
+ * moments).  dWFD (optional, may be NULL): [ncell][4][N][N] Pauli components of the fitted states. */
+int qbf_fermi_dirac(qbf_engine *e, const double *dW, int ncell, double L, double *dParams, double *dWFD);
+
+/* distance and quantum relative entropy to Fermi-Dirac states (visualizeHomBoltzmann.m:120-147,
+ * visualizeInhomBoltzmann.m:163-205): dOut[ncell][2] = sum_j ||W_j - WFD_j||_1 dv (matrix 1-norm,
+ * maximum over v_x of the sum over v_y), sum_v S(W(v) || WFD(v)) dv over WFD_0(v) > 1e-6.
+ * fd_stride 0: one state dWFD for all cells; 1: one per cell.  N <= 64. */
+int qbf_fd_distance(qbf_engine *e, const double *dW, int ncell, double L, const double *dWFD, int fd_stride,
+	double *dOut);
+
+/* average state over the cells, summed in cell order (visualizeInhomBoltzmann.m:80-89) */
+int qbf_cell_average(qbf_engine *e, const double *dW, int ncell, double *dOut);
+
 /* Timing of the dominant kernel (the I3 K-operator kernel): qbf_profile(e, 1) clears and
  * starts recording HIP events around each of its launches on the engine stream;
  * qbf_profile_read returns the summed kernel time, the launch count and the number of

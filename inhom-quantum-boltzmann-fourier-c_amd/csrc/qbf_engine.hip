@@ -954,6 +954,34 @@ int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, d
 	return 0;
 }
 
+int qbf_fermi_dirac(qbf_engine* e, const double* dW, int ncell, double L, double* dParams, double* dWFD)
+{
+	if (!e || !dW || !dParams || ncell < 1) return set_err(-1, "qbf_fermi_dirac: bad argument");
+	hipLaunchKernelGGL(k_fermi_dirac, dim3(ncell), dim3(256), 0, e->stream, dW, e->gridN, L, dParams, dWFD);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int qbf_fd_distance(qbf_engine* e, const double* dW, int ncell, double L, const double* dWFD, int fd_stride,
+	double* dOut)
+{
+	if (!e || !dW || !dWFD || !dOut || ncell < 1 || (fd_stride != 0 && fd_stride != 1))
+		return set_err(-1, "qbf_fd_distance: bad argument");
+	if (e->gridN > 64) return set_err(-1, "qbf_fd_distance: N = %d above 64", e->gridN);
+	hipLaunchKernelGGL(k_fd_distance, dim3(ncell), dim3(256), 0, e->stream, dW, dWFD, fd_stride, e->gridN, L, dOut);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int qbf_cell_average(qbf_engine* e, const double* dW, int ncell, double* dOut)
+{
+	if (!e || !dW || !dOut || ncell < 1) return set_err(-1, "qbf_cell_average: bad argument");
+	const long long nel = 4LL * e->gridN * e->gridN;
+	hipLaunchKernelGGL(k_cell_average, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, e->stream, dW, ncell, nel, dOut);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
 int qbf_moment_correct(qbf_engine* e, const double* Wold, double* Wnew, int ncell, double L, int mask)
 {
 	if (!e || !Wold || !Wnew || ncell < 1 || mask < 0 || mask > 7) return set_err(-1, "bad argument");

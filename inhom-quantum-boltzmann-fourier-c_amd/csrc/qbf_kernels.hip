@@ -743,6 +743,206 @@ __global__ __launch_bounds__(256) void k_cell_diag(const double* __restrict__ W,
 	if (threadIdx.x == 0) out[(size_t)blockIdx.x * 10 + 9] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fermi-Dirac equilibrium diagnostics (visualization/asymptoticFD.m, calculateFDparams.m,
+// createFD.m, relentropy.m, visualizeHomBoltzmann.m:114-149, visualizeInhomBoltzmann.m:163-205).
+
+// Dilogarithm Li2(x) for x <= 0.  For -1 <= x <= 0 the Bernoulli series in u = -ln(1 - x),
+// Li2(x) = sum_k B_k u^(k+1) / (k+1)!  (|u| <= ln 2, terms up to k = 20 reach double precision);
+// for x < -1 the inversion Li2(x) = -pi^2/6 - ln^2(-x)/2 - Li2(1/x).  ('t Hooft and Veltman,
+// Nucl. Phys. B153 (1979) 365.)
+__device__ double li2_nonpos(double x)
+{
+	double add = 0.0, sgn = 1.0;
+	if (x < -1.0) {
+		const double l = log(-x);
+		add = -M_PI * M_PI / 6.0 - 0.5 * l * l;
+		sgn = -1.0;
+		x = 1.0 / x;
+	}
+	const double u = -log1p(-x), u2 = u * u;
+	// B_k / (k+1)! for k = 2, 4, ..., 20
+	constexpr double c[10] = {1.0 / 36.0, -1.0 / 3600.0, 1.0 / 211680.0, -1.0 / 10886400.0, 1.0 / 526901760.0,
+		-4.0647616451442256e-11, 8.921691020456452e-13, -1.9939295860721074e-14, 4.518980029619918e-16,
+		-1.0356517612181247e-17};
+	double t = c[9];
+	for (int i = 8; i >= 0; i--) t = c[i] + u2 * t;
+	return add + sgn * (u - 0.25 * u2 + u * u2 * t);
+}
+
+// Energy of the two-species 2D Fermi-Dirac gas with densities l1, l2 at inverse temperature beta
+// (calculateFDparams.m:65-83): the density equations 2 pi / beta ln(1 + z) = l fix the fugacities,
+// z = expm1(beta l / 2 pi), and E = -2 pi / beta^2 (Li2(-z1) + Li2(-z2)).
+__device__ double fd_energy(double beta, double l1, double l2)
+{
+	const double k = beta / (2.0 * M_PI);
+	return -(2.0 * M_PI) / (beta * beta) * (li2_nonpos(-expm1(l1 * k)) + li2_nonpos(-expm1(l2 * k)));
+}
+
+// beta with fd_energy(beta) = E.  E decreases monotonically from +inf (beta -> 0) to the ground
+// state (l1^2 + l2^2) / 4 pi (beta -> inf), so a bracket in beta and bisection in log(beta) find
+// the unique root; returns 0 (status 1) when there is none.
+__device__ double fd_solve_beta(double l1, double l2, double E, int* status)
+{
+	*status = 1;
+	if (!(l1 > 0.0) || !(l2 >= l1) || !(E > (l1 * l1 + l2 * l2) / (4.0 * M_PI)) || !(E < INFINITY)) return 0.0;
+	const double b0 = (l1 + l2) / E;   // classical limit, exact as beta -> 0
+	double lo = b0, hi = b0;
+	for (int i = 0; i < 1100 && fd_energy(lo, l1, l2) < E; i++) lo *= 0.5;
+	for (int i = 0; i < 1100 && fd_energy(hi, l1, l2) > E; i++) hi *= 2.0;
+	if (!(fd_energy(lo, l1, l2) >= E) || !(fd_energy(hi, l1, l2) <= E)) return 0.0;
+	for (int it = 0; it < 200 && hi > lo * (1.0 + 4e-16); it++) {
+		const double mid = sqrt(lo * hi);
+		if (fd_energy(mid, l1, l2) > E) lo = mid;
+		else hi = mid;
+	}
+	*status = 0;
+	return sqrt(lo * hi);
+}
+
+// Per cell: the Fermi-Dirac state with the cell's conserved moments (asymptoticFD.m).  Moments as
+// wignerMoments.m:15-37 (spin density matrix rho = w0 + w.sigma, velocity, intrinsic energy about
+// it); eigenvalues l1 <= l2 of rho with eigenvectors -n, +n; then beta, mu_i = ln(z_i) / beta and
+// W_FD(v) = f1 P- + f2 P+, f_i = 1 / (exp(beta (|v - vel|^2 / 2 - mu_i)) + 1), P+- = (1 +- n.sigma) / 2,
+// i.e. Pauli components ((f1 + f2) / 2, (f2 - f1) / 2 n).
+// params[cell][12] = beta, mu1, mu2, vel_x, vel_y, n_x, n_y, n_z, l1, l2, en, status (0: fitted,
+// 1: no Fermi-Dirac state has these moments, W_FD = 0).
+__global__ __launch_bounds__(256) void k_fermi_dirac(const double* __restrict__ W, int N, double L,
+	double* __restrict__ params, double* __restrict__ WFD)
+{
+	__shared__ double sh[4];
+	__shared__ double prm[12];
+	const int NN = N * N;
+	const double* w = W + (size_t)blockIdx.x * 4 * NN;
+	const double sc = (2 * L) / N, dv = sc * sc;
+	double acc[6] = {0, 0, 0, 0, 0, 0};
+	for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+		const int l = e / N, k = e % N;
+		const double vx = l < N / 2 ? sc * l : sc * (l - N);
+		const double vy = k < N / 2 ? sc * k : sc * (k - N);
+		const double w0 = w[e];
+		acc[0] += w0;
+		acc[1] += w[NN + e];
+		acc[2] += w[2 * NN + e];
+		acc[3] += w[3 * NN + e];
+		acc[4] += vx * w0;
+		acc[5] += vy * w0;
+	}
+	double tot[6];
+	for (int i = 0; i < 6; i++) tot[i] = block_sum_256(acc[i], sh);
+	const double velx = tot[4] / tot[0], vely = tot[5] / tot[0];
+	double ein = 0;
+	for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+		const int l = e / N, k = e % N;
+		const double dx = (l < N / 2 ? sc * l : sc * (l - N)) - velx;
+		const double dy = (k < N / 2 ? sc * k : sc * (k - N)) - vely;
+		ein += 0.5 * (dx * dx + dy * dy) * w[e];
+	}
+	ein = block_sum_256(ein, sh);
+	if (threadIdx.x == 0) {
+		const double m0 = tot[0] * dv, mx = tot[1] * dv, my = tot[2] * dv, mz = tot[3] * dv;
+		const double r = sqrt(mx * mx + my * my + mz * mz);
+		const double l1 = m0 - r, l2 = m0 + r;
+		const double en = ein / tot[0];                 // normalised intrinsic energy
+		int st;
+		const double beta = fd_solve_beta(l1, l2, en * (l1 + l2), &st);
+		const double k = beta / (2.0 * M_PI);
+		prm[0] = beta;
+		prm[1] = st ? 0.0 : log(expm1(l1 * k)) / beta;
+		prm[2] = st ? 0.0 : log(expm1(l2 * k)) / beta;
+		prm[3] = velx;
+		prm[4] = vely;
+		prm[5] = r > 0 ? mx / r : 0.0;
+		prm[6] = r > 0 ? my / r : 0.0;
+		prm[7] = r > 0 ? mz / r : 0.0;
+		prm[8] = l1;
+		prm[9] = l2;
+		prm[10] = en;
+		prm[11] = st;
+	}
+	__syncthreads();
+	if (threadIdx.x < 12) params[(size_t)blockIdx.x * 12 + threadIdx.x] = prm[threadIdx.x];
+	if (!WFD) return;
+	double* o = WFD + (size_t)blockIdx.x * 4 * NN;
+	const bool ok = prm[11] == 0.0;
+	for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+		const int l = e / N, k = e % N;
+		const double dx = (l < N / 2 ? sc * l : sc * (l - N)) - prm[3];
+		const double dy = (k < N / 2 ? sc * k : sc * (k - N)) - prm[4];
+		const double ek = 0.5 * (dx * dx + dy * dy);
+		const double f1 = ok ? 1.0 / (exp(prm[0] * (ek - prm[1])) + 1.0) : 0.0;
+		const double f2 = ok ? 1.0 / (exp(prm[0] * (ek - prm[2])) + 1.0) : 0.0;
+		const double h = 0.5 * (f2 - f1);
+		o[e] = 0.5 * (f1 + f2);
+		o[NN + e] = h * prm[5];
+		o[2 * NN + e] = h * prm[6];
+		o[3 * NN + e] = h * prm[7];
+	}
+}
+
+// Quantum relative entropy S(rho || tau) of two 2x2 density matrices given by Pauli components
+// (relentropy.m): eigenvalues clamped to [eps, 1 - eps] (rho is replaced by its clamped form),
+// tr rho (ln rho - ln tau) + tr (1 - rho)(ln(1 - rho) - ln(1 - tau)), in closed form: for
+// A = a + b.sigma and B = c + d.sigma, tr AB = 2 (a c + b.d).
+__device__ double relentropy2(double a, double b1, double b2, double b3, double c, double d1, double d2, double d3)
+{
+	const double eps = 2.220446049250313e-16;
+	auto clampe = [&](double x) { return x <= 0.0 ? eps : x >= 1.0 ? 1.0 - eps : x; };
+	const double rb = sqrt(b1 * b1 + b2 * b2 + b3 * b3), rd = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+	const double mp = clampe(a + rb), mm = clampe(a - rb), np_ = clampe(c + rd), nm = clampe(c - rd);
+	const double cosbd = (rb > 0 && rd > 0) ? (b1 * d1 + b2 * d2 + b3 * d3) / (rb * rd) : 0.0;
+	const double am = 0.5 * (mp + mm), bm = 0.5 * (mp - mm);          // clamped rho = am + bm b^.sigma
+	const double lt0 = 0.5 * (log(np_) + log(nm)), lt1 = 0.5 * (log(np_) - log(nm));
+	const double lu0 = 0.5 * (log1p(-np_) + log1p(-nm)), lu1 = 0.5 * (log1p(-np_) - log1p(-nm));
+	const double rr = mp * log(mp) + mm * log(mm) + (1 - mp) * log1p(-mp) + (1 - mm) * log1p(-mm);
+	const double rt = 2.0 * (am * lt0 + bm * lt1 * cosbd);             // tr rho ln tau
+	const double ru = 2.0 * ((1 - am) * lu0 - bm * lu1 * cosbd);       // tr (1 - rho) ln(1 - tau)
+	return rr - rt - ru;
+}
+
+// Per cell: distance and relative entropy to a Fermi-Dirac state (visualizeHomBoltzmann.m:120-147):
+// out[cell][0] = sum_j ||W_j - WFD_j||_1 dv with the matrix 1-norm of the N x N grid (maximum over
+// the slow index l of sum_k |.|, as MATLAB's norm(A, 1) on the [k, l] matrix), out[cell][1] =
+// sum_v S(W(v) || WFD(v)) dv over the velocities with WFD_0(v) > 1e-6.  fd_stride 0: one state WFD
+// for every cell (the global equilibrium); 1: one per cell.  N <= 64.
+__global__ __launch_bounds__(256) void k_fd_distance(const double* __restrict__ W, const double* __restrict__ WFD,
+	int fd_stride, int N, double L, double* __restrict__ out)
+{
+	__shared__ double sh[4];
+	const int NN = N * N;
+	const double* w = W + (size_t)blockIdx.x * 4 * NN;
+	const double* f = WFD + (size_t)blockIdx.x * fd_stride * 4 * NN;
+	const double sc = (2 * L) / N, dv = sc * sc;
+	// one wave per component, one lane per column l
+	const int j = threadIdx.x >> 6, l = threadIdx.x & 63;
+	double col = 0;
+	if (l < N)
+		for (int k = 0; k < N; k++) col += fabs(w[j * NN + k + N * l] - f[j * NN + k + N * l]);
+	for (int o = 32; o > 0; o >>= 1) col = fmax(col, __shfl_xor(col, o));
+	double dist = (l == 0) ? col : 0.0;
+	dist = block_sum_256(dist, sh);
+	double kl = 0;
+	for (int e = threadIdx.x; e < NN; e += blockDim.x)
+		if (f[e] > 1e-6)
+			kl += relentropy2(w[e], w[NN + e], w[2 * NN + e], w[3 * NN + e], f[e], f[NN + e], f[2 * NN + e], f[3 * NN + e]);
+	kl = block_sum_256(kl, sh);
+	if (threadIdx.x == 0) {
+		out[(size_t)blockIdx.x * 2] = dist * dv;
+		out[(size_t)blockIdx.x * 2 + 1] = kl * dv;
+	}
+}
+
+// Average over the cells, summed in cell order (visualizeInhomBoltzmann.m:80-89).
+__global__ __launch_bounds__(256) void k_cell_average(const double* __restrict__ W, int ncell, long long nel,
+	double* __restrict__ out)
+{
+	const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= nel) return;
+	double s = 0.0;
+	for (int c = 0; c < ncell; c++) s += W[(size_t)c * nel + e];
+	out[e] = s / ncell;
+}
+
 // y = a * x + b * y  (elementwise, double)
 __global__ void k_axpby(const double* __restrict__ x, double* __restrict__ y, double a, double b, long long n)
 {

@@ -84,6 +84,9 @@ def lib():
             "qbf_set_moment_correction": (i, [vp, i]),
             "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
             "qbf_cell_diagnostics": (i, [vp, vp, i, d, vp]),
+            "qbf_fermi_dirac": (i, [vp, vp, i, d, vp, vp]),
+            "qbf_fd_distance": (i, [vp, vp, i, d, vp, i, vp]),
+            "qbf_cell_average": (i, [vp, vp, i, vp]),
             "qbf_simulate_periodic": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
             "qbf_profile": (i, [vp, i]),
             "qbf_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]),
@@ -200,6 +203,34 @@ class Engine:
         self._bind()
         out = self._torch.empty((W.shape[0], 10), dtype=self._torch.float64, device=W.device)
         _check(lib().qbf_cell_diagnostics(self._h, _ptr(W), W.shape[0], L, _ptr(out)))
+        return out
+
+    def fermi_dirac(self, W, L, with_states=True):
+        """Fermi-Dirac equilibria with the conserved moments of each state of W [n, 4, N, N]:
+        (params [n, 12]: beta, mu1, mu2, vel_x, vel_y, n_x, n_y, n_z, lambda1, lambda2, en, status;
+        states [n, 4, N, N] or None)."""
+        self._bind()
+        t = self._torch
+        prm = t.empty((W.shape[0], 12), dtype=t.float64, device=W.device)
+        wfd = t.empty_like(W) if with_states else None
+        _check(lib().qbf_fermi_dirac(self._h, _ptr(W), W.shape[0], L, _ptr(prm), _ptr(wfd)))
+        return prm, wfd
+
+    def fd_distance(self, W, L, WFD):
+        """[n, 2]: L1 distance (matrix 1-norm per component) and quantum relative entropy of each
+        state of W to WFD: one state [4, N, N] (or [1, 4, N, N]) for all, or one per state."""
+        self._bind()
+        t = self._torch
+        stride = 1 if (WFD.dim() == 4 and WFD.shape[0] == W.shape[0] and W.shape[0] > 1) else 0
+        out = t.empty((W.shape[0], 2), dtype=t.float64, device=W.device)
+        _check(lib().qbf_fd_distance(self._h, _ptr(W), W.shape[0], L, _ptr(WFD.contiguous()), stride, _ptr(out)))
+        return out
+
+    def cell_average(self, W):
+        """[1, 4, N, N]: the average of the states of W over the cells."""
+        self._bind()
+        out = self._torch.empty((1,) + tuple(W.shape[1:]), dtype=W.dtype, device=W.device)
+        _check(lib().qbf_cell_average(self._h, _ptr(W), W.shape[0], _ptr(out)))
         return out
 
     # --------------------------------------------------------------- stream
