@@ -131,6 +131,16 @@ int qbf_moment_correct(qbf_engine *e, const double *dWold, double *dWnew, int nc
  * (visualization/wignerMoments.m:15-37, entropy.m, visualizeHomBoltzmann.m:26-45,98-105). */
 int qbf_cell_diagnostics(qbf_engine *e, const double *dW, int ncell, double L, double *dOut);
 
+/* Paired I3 path (no counterpart in the reference; an option of the FP32 convolution path).  For
+ * spectra of real states every grid that K_q = FFT2(psi2 .* IFFT2(.)) acts on is real, so one complex
+ * application carries two; the imaginary parts it drops come from the Nyquist lines of the state
+ * (zero padding places them without their conjugate partners).  mode 0: off; 1: on when the
+ * measured max |Im| / max |Re| of the padded physical grids of the state is at most tol; 2: always.
+ * Defaults: FP64 engines 0, FP32 engines 1 with tol = 1e-6.  qbf_real_pairs_status: whether the last collision evaluation used it, and the ratio
+ * measured (-1 when not measured). */
+int qbf_set_real_pairs(qbf_engine *e, int mode, double tol);
+int qbf_real_pairs_status(qbf_engine *e, int *used, double *ratio);
+
 /* Fermi-Dirac equilibrium with each cell's conserved moments (visualization/asymptoticFD.m,
  * calculateFDparams.m, createFD.m).  dParams[ncell][12] = beta, mu1, mu2, vel_x, vel_y, n_x, n_y, n_z
  * (eigenvector of the larger eigenvalue of the spin density matrix), lambda1 <= lambda2 (its

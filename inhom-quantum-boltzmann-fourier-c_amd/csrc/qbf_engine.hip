@@ -76,6 +76,12 @@ struct qbf_engine {
 	}
 	int gridN = 0, precision = 0, maxCells = 0;
 	int momentMask = 0;               // conservation correction after each collision step (0 = off)
+	// paired I3 path: 0 off, 1 when the measured imaginary/real ratio of the padded physical grids
+	// is at most pairTol, 2 always; pairRatio / pairUsed: the last collision evaluation's measurement
+	// and choice
+	int pairMode = 0;
+	double pairTol = 0.0, pairRatio = -1.0;
+	int pairUsed = 0;
 	// optional timing of the I3 kernel (k_i3acc) launches with HIP events on the engine stream
 	bool prof = false;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
@@ -199,6 +205,7 @@ struct EngineT final : public qbf_engine {
 
 	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, terms, xscr;
 	DevBuf Wmid, WF, WF2, Wt1, Wt2, src3;
+	DevBuf Wpk, WhatP, reim;          // paired I3 path: packed spectra, their padded transforms, realness
 	size_t fBudget = (size_t)2 << 30;
 
 	int init()
@@ -356,7 +363,7 @@ struct EngineT final : public qbf_engine {
 	// I3 over all quadrature terms; returns the number of accumulator chunks per (cell, out)
 	int run_i3(const double2* src, int nsrc, int nF, const int* srcOfF, const void* Shat, int nS, int nout,
 		const int* ninst, const int (*iF)[4], const int (*iS)[4], const int (*iH)[4], const double (*iW)[4],
-		int ncell, int* nchunkOut)
+		int ncell, int* nchunkOut, bool pair = false)
 	{
 		if (n3 <= 0) return set_err(-1, "I3 table not set");
 		const int nq = n3;
@@ -381,15 +388,20 @@ struct EngineT final : public qbf_engine {
 			for (int o = 0; o < nout; o++) {
 				a.ninst[o] = ninst[o];
 				for (int i = 0; i < ninst[o]; i++) {
-					a.instF[o][i] = iF[o][i]; a.instS[o][i] = iS[o][i]; a.instH[o][i] = iH[o][i]; a.instW[o][i] = iW[o][i];
+					a.instF[o][i] = iF[o][i]; a.instS[o][i] = iS[o][i]; a.instH[o][i] = iH[o][i];
 				}
+				// paired instances carry two weights each
+				for (int i = 0; i < (pair ? 2 * ninst[o] : ninst[o]); i++) a.instW[o][i] = iW[o][i];
 			}
 			a.nchunk = nchunk; a.accumulate = q0 > 0; a.acc = acc3.p; a.tw256 = g_tw256;
 			r = xs_for(ncell * nout * nchunk, &a.xs);
 			if (r) return r;
 			hipEvent_t eb = nullptr, ee = nullptr;
 			if (prof && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
-			hipLaunchKernelGGL((k_i3acc<R, N, T, XG>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			if (pair)
+				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, true>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			else
+				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, false>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			HIPCHK(hipGetLastError());
 			if (ee) {
 				HIPCHK(hipEventRecord(ee, stream));
@@ -400,6 +412,56 @@ struct EngineT final : public qbf_engine {
 		}
 		*nchunkOut = nchunk;
 		return 0;
+	}
+
+	// max |Im| / max |Re| over the padded physical grids of the state (What): the size of what the
+	// paired I3 path drops, relative to what it keeps
+	int measure_realness(int ncell, double* ratio)
+	{
+		constexpr int NB = 256;
+		int r = reim.ensure(sizeof(double) * 2 * NB);
+		if (r) return r;
+		hipLaunchKernelGGL((k_re_im_max<R>), dim3(NB), dim3(256), 0, stream, What.template as<C>(),
+			(long long)ncell * 4 * PP, reim.template as<double>());
+		HIPCHK(hipGetLastError());
+		double h[2 * NB];
+		HIPCHK(hipMemcpyAsync(h, reim.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+		HIPCHK(hipStreamSynchronize(stream));
+		double mr = 0, mi = 0;
+		for (int b = 0; b < NB; b++) {
+			mr = fmax(mr, h[2 * b]);
+			mi = fmax(mi, h[2 * b + 1]);
+		}
+		*ratio = mr > 0 ? mi / mr : (mi > 0 ? INFINITY : 0.0);
+		return 0;
+	}
+
+	// I3 part of CdInt on packed component pairs (see k_i3acc, PAIR)
+	int run_i3_pairs(const double2* W, int ncell, int* nchunk)
+	{
+		int r = Wpk.ensure(sizeof(double2) * 2 * NN * (size_t)ncell);
+		if (!r) r = WhatP.ensure((size_t)ncell * 2 * GB);
+		if (r) return r;
+		const long long n = 2LL * NN * ncell;
+		hipLaunchKernelGGL(k_pack_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, W,
+			Wpk.template as<double2>(), NN, n);
+		HIPCHK(hipGetLastError());
+		const int two[2] = {0, 1};
+		r = launch_padfft(Wpk.template as<double2>(), 2, nullptr, 0, 0, 1, 2, two, WhatP.p, ncell);
+		if (r) return r;
+		static const double eta[4] = {1, -1, -1, -1};
+		int ni[4] = {2, 2, 2, 2};
+		int iF[4][4] = {{0}}, iS[4][4] = {{0}}, iH[4][4] = {{0}};
+		double iW[4][4] = {{0}};
+		for (int j = 0; j < 4; j++)
+			for (int pr = 0; pr < 2; pr++) {
+				iF[j][pr] = pr;
+				iS[j][pr] = pr;
+				iH[j][pr] = 2 * (j >> 1) + (j & 1);
+				iW[j][2 * pr] = eta[2 * pr];
+				iW[j][2 * pr + 1] = eta[2 * pr + 1];
+			}
+		return run_i3(Wpk.template as<double2>(), 2, 2, two, WhatP.p, 2, 4, ni, iF, iS, iH, iW, ncell, nchunk, true);
 	}
 
 	int launch_combine(std::vector<CTerm> (&tl)[4], int nout, const int* outComp, int ncell, double2* out,
@@ -463,13 +525,24 @@ struct EngineT final : public qbf_engine {
 			r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat.p, ncell);
 			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf.p, ncell);
 			if (r) return r;
-			static const double eta[4] = {1, -1, -1, -1};
-			int ni[4] = {4, 4, 4, 4};
-			int iF[4][4], iS[4][4], iH[4][4];
-			double iW[4][4];
-			for (int j = 0; j < 4; j++)
-				for (int i = 0; i < 4; i++) { iF[j][i] = i; iS[j][i] = i; iH[j][i] = j; iW[j][i] = eta[i]; }
-			r = run_i3(W, 4, 4, all4, What.p, 4, 4, ni, iF, iS, iH, iW, ncell, &nchunk);
+			pairUsed = 0;
+			pairRatio = -1.0;
+			if (pairMode == 1) {
+				r = measure_realness(ncell, &pairRatio);
+				if (r) return r;
+			}
+			if (pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol)) {
+				pairUsed = 1;
+				r = run_i3_pairs(W, ncell, &nchunk);
+			} else {
+				static const double eta[4] = {1, -1, -1, -1};
+				int ni[4] = {4, 4, 4, 4};
+				int iF[4][4], iS[4][4], iH[4][4];
+				double iW[4][4];
+				for (int j = 0; j < 4; j++)
+					for (int i = 0; i < 4; i++) { iF[j][i] = i; iS[j][i] = i; iH[j][i] = j; iW[j][i] = eta[i]; }
+				r = run_i3(W, 4, 4, all4, What.p, 4, 4, ni, iF, iS, iH, iW, ncell, &nchunk);
+			}
 			if (r) return r;
 		}
 		const double p2 = 1.0 / ((double)P * P), p4 = p2 * p2;
@@ -696,6 +769,11 @@ int make_engine(int maxCells, qbf_engine** out)
 	e->gridN = N;
 	e->precision = std::is_same<R, float>::value ? QBF_FP32 : QBF_FP64;
 	e->maxCells = maxCells;
+	if (std::is_same<R, float>::value) {
+		// FP32 grids carry ~6e-8 relative rounding already: pair when the dropped part is below 1e-6
+		e->pairMode = 1;
+		e->pairTol = 1e-6;
+	}
 	int r = e->init();
 	if (r) {
 		delete e;
@@ -951,6 +1029,22 @@ int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, d
 	if (!e || !dW || !dOut || ncell < 1) return set_err(-1, "qbf_cell_diagnostics: bad argument");
 	hipLaunchKernelGGL(k_cell_diag, dim3(ncell), dim3(256), 0, e->stream, dW, e->gridN, L, dOut);
 	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int qbf_set_real_pairs(qbf_engine* e, int mode, double tol)
+{
+	if (!e || mode < 0 || mode > 2 || !(tol >= 0)) return set_err(-1, "qbf_set_real_pairs: bad argument");
+	e->pairMode = mode;
+	e->pairTol = tol;
+	return 0;
+}
+
+int qbf_real_pairs_status(qbf_engine* e, int* used, double* ratio)
+{
+	if (!e) return set_err(-1, "qbf_real_pairs_status: null engine");
+	if (used) *used = e->pairUsed;
+	if (ratio) *ratio = e->pairRatio;
 	return 0;
 }
 

@@ -190,7 +190,13 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 
 // ---------------------------------------------------------------------------------------------
 // The dominant kernel: 16 applications of K_q per quadrature term and cell for CdInt.
-template <typename R, int N, int T, bool XG>
+// PAIR: the grids hold packed pairs of real grids (real part: component 2p, imaginary part:
+// component 2p+1), S and F from the packed spectra W_2p + i W_2p+1 and H one component of a packed
+// F grid (instH = 2 slot + component).  K_q maps real grids to real grids, so one complex K_q
+// application carries the two instances (S_2p H, S_2p+1 H) and acc += w_2p F_2p Y_2p + w_2p+1 F_2p+1 Y_2p+1
+// (instW[out][2 in], [2 in + 1]).  Exact for spectra of real states whose Nyquist lines vanish;
+// otherwise off by the dropped imaginary parts, which the host bounds before choosing it.
+template <typename R, int N, int T, bool XG, bool PAIR>
 __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 {
 	using GE = Geo<2 * N, T>;
@@ -224,11 +230,22 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 		const R* ps = (const R*)a.psi2n + (size_t)(a.q0 + qq) * (P * P);
 		for (int in = 0; in < ni; in++) {
 			const cx<R>* Sg = Sh + (size_t)a.instS[out][in] * (P * P);
-			const cx<R>* Fh = Fq + (size_t)a.instH[out][in] * (P * P);
+			const int hs = a.instH[out][in];
+			const cx<R>* Fh = Fq + (size_t)(PAIR ? hs >> 1 : hs) * (P * P);
 			const cx<R>* Ff = Fq + (size_t)a.instF[out][in] * (P * P);
-			const R w = (R)a.instW[out][in];
+			if constexpr (PAIR) {
+				const bool hy = hs & 1;
 #pragma unroll
-			for (int u = 0; u < E; u++) v[u] = cmul<R>(uni(Sg, (long long)u * T)[t], uni(Fh, (long long)u * T)[t]);
+				for (int u = 0; u < E; u++) {
+					const cx<R> h = uni(Fh, (long long)u * T)[t];
+					const R hv = hy ? h.y : h.x;
+					const cx<R> sv = uni(Sg, (long long)u * T)[t];
+					v[u] = mk<R>(sv.x * hv, sv.y * hv);
+				}
+			} else {
+#pragma unroll
+				for (int u = 0; u < E; u++) v[u] = cmul<R>(uni(Sg, (long long)u * T)[t], uni(Fh, (long long)u * T)[t]);
+			}
 			ifft2_reg<R, GE>(v, X, tw, line, p);
 			// keep the table and operand loads next to their use: hoisting them above the
 			// line FFTs would keep E extra complex values live and spill the accumulator
@@ -241,11 +258,21 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 			}
 			fft2_reg<R, GE>(v, X, tw, line, p);
 			__builtin_amdgcn_sched_barrier(0);
+			if constexpr (PAIR) {
+				const R wa = (R)a.instW[out][2 * in], wb = (R)a.instW[out][2 * in + 1];
 #pragma unroll
-			for (int u = 0; u < E; u++) {
-				const cx<R> y = cmul<R>(uni(Ff, (long long)u * T)[t], v[u]);
-				acc[u].x += w * y.x;
-				acc[u].y += w * y.y;
+				for (int u = 0; u < E; u++) {
+					const cx<R> f = uni(Ff, (long long)u * T)[t];
+					acc[u].x += (wa * f.x) * v[u].x + (wb * f.y) * v[u].y;
+				}
+			} else {
+				const R w = (R)a.instW[out][in];
+#pragma unroll
+				for (int u = 0; u < E; u++) {
+					const cx<R> y = cmul<R>(uni(Ff, (long long)u * T)[t], v[u]);
+					acc[u].x += w * y.x;
+					acc[u].y += w * y.y;
+				}
 			}
 			__builtin_amdgcn_sched_barrier(0);
 		}
@@ -741,6 +768,49 @@ __global__ __launch_bounds__(256) void k_cell_diag(const double* __restrict__ W,
 	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = lmax;
 	__syncthreads();
 	if (threadIdx.x == 0) out[(size_t)blockIdx.x * 10 + 9] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Packed component pairs for the paired I3 path: Wp[cell][p] = W[cell][2p] + i W[cell][2p+1]
+// (n = ncell * 2 * NN elements).  The padded transform of Wp (and of psi1 .* Wp) is the pair of
+// transforms of the two components in its real and imaginary parts when both are real.
+__global__ __launch_bounds__(256) void k_pack_pairs(const double2* __restrict__ W, double2* __restrict__ Wp, int NN,
+	long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const long long cell = i / (2 * NN);
+	const int r = (int)(i % (2 * NN));
+	const int pr = r / NN, e = r % NN;
+	const double2 x = W[(cell * 4 + 2 * pr) * NN + e], y = W[(cell * 4 + 2 * pr + 1) * NN + e];
+	Wp[i] = make_double2(x.x - y.y, x.y + y.x);
+}
+
+// Largest |real part| and |imaginary part| of n complex values: out[block] = (max |re|, max |im|)
+// over the block's grid-stride share.
+template <typename R>
+__global__ __launch_bounds__(256) void k_re_im_max(const cx<R>* __restrict__ g, long long n, double* __restrict__ out)
+{
+	__shared__ double sh[2][4];
+	double mr = 0, mi = 0;
+	for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+		const cx<R> z = g[i];
+		mr = fmax(mr, fabs((double)z.x));
+		mi = fmax(mi, fabs((double)z.y));
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		mr = fmax(mr, __shfl_xor(mr, o));
+		mi = fmax(mi, __shfl_xor(mi, o));
+	}
+	if ((threadIdx.x & 63) == 0) {
+		sh[0][threadIdx.x >> 6] = mr;
+		sh[1][threadIdx.x >> 6] = mi;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		out[2 * blockIdx.x] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+		out[2 * blockIdx.x + 1] = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
+	}
 }
 
 // ---------------------------------------------------------------------------------------------

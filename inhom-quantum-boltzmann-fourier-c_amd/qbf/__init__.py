@@ -85,6 +85,8 @@ def lib():
             "qbf_moment_correct": (i, [vp, vp, vp, i, d, i]),
             "qbf_cell_diagnostics": (i, [vp, vp, i, d, vp]),
             "qbf_fermi_dirac": (i, [vp, vp, i, d, vp, vp]),
+            "qbf_set_real_pairs": (i, [vp, i, d]),
+            "qbf_real_pairs_status": (i, [vp, C.POINTER(i), C.POINTER(d)]),
             "qbf_fd_distance": (i, [vp, vp, i, d, vp, i, vp]),
             "qbf_cell_average": (i, [vp, vp, i, vp]),
             "qbf_simulate_periodic": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
@@ -204,6 +206,16 @@ class Engine:
         out = self._torch.empty((W.shape[0], 10), dtype=self._torch.float64, device=W.device)
         _check(lib().qbf_cell_diagnostics(self._h, _ptr(W), W.shape[0], L, _ptr(out)))
         return out
+
+    def set_real_pairs(self, mode: int, tol: float = 0.0):
+        """Paired I3 path: 0 off, 1 when the measured imaginary/real ratio <= tol, 2 always."""
+        _check(lib().qbf_set_real_pairs(self._h, mode, tol))
+
+    def real_pairs_status(self):
+        """(used, ratio) of the last collision evaluation."""
+        u, r = C.c_int(0), C.c_double(0)
+        _check(lib().qbf_real_pairs_status(self._h, C.byref(u), C.byref(r)))
+        return bool(u.value), r.value
 
     def fermi_dirac(self, W, L, with_states=True):
         """Fermi-Dirac equilibria with the conserved moments of each state of W [n, 4, N, N]:
