@@ -770,9 +770,10 @@ int make_engine(int maxCells, qbf_engine** out)
 	e->precision = std::is_same<R, float>::value ? QBF_FP32 : QBF_FP64;
 	e->maxCells = maxCells;
 	if (std::is_same<R, float>::value) {
-		// FP32 grids carry ~6e-8 relative rounding already: pair when the dropped part is below 1e-6
+		// the paired path deviates from the complex one by about 0.15-0.8 times the measured ratio;
+		// below 1e-5 that stays under the FP32 path's own deviation from FP64 (2e-5 at N = 64)
 		e->pairMode = 1;
-		e->pairTol = 1e-6;
+		e->pairTol = 1e-5;
 	}
 	int r = e->init();
 	if (r) {
