@@ -137,7 +137,7 @@ int qbf_cell_diagnostics(qbf_engine *e, const double *dW, int ncell, double L, d
  * (zero padding places them without their conjugate partners).  mode 0: off; 1: on when the
  * measured max |Im| / max |Re| of the padded physical grids of the state is at most tol; 2: always.
  * The paired result deviates from the complex one by about 0.15-0.8 times the ratio.  Defaults:
- * FP64 engines 0, FP32 engines 1 with tol = 1e-5.  qbf_real_pairs_status: whether the last collision evaluation used it, and the ratio
+ * FP64 engines 0, FP32 engines 1 with tol = 2e-5.  qbf_real_pairs_status: whether the last collision evaluation used it, and the ratio
  * measured (-1 when not measured). */
 int qbf_set_real_pairs(qbf_engine *e, int mode, double tol);
 int qbf_real_pairs_status(qbf_engine *e, int *used, double *ratio);

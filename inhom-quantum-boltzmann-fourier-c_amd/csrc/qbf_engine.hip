@@ -328,35 +328,56 @@ struct EngineT final : public qbf_engine {
 		return 0;
 	}
 
+	// Term ranges per (cell, instance) so that a launch has about 1024 workgroups when there are few
+	// cells, each range keeping at least minTerms terms; the partial sums land in consecutive slots.
+	static int term_chunks(int ncell, int ninst, int nterm, int minTerms)
+	{
+		int c = (1024 + ncell * ninst - 1) / (ncell * ninst);
+		const int cmax = nterm / minTerms;
+		if (c > cmax) c = cmax;
+		return c < 1 ? 1 : c;
+	}
+
 	int launch_i1(int slot, const double2* src, int nsrc, const void* Fhat, int nFh, double fa, double fb,
-		int ninst, const int* instSrc, const int* instFh, void* out, int ncell)
+		int ninst, const int* instSrc, const int* instFh, DevBuf& out, int ncell, int* nchunk)
 	{
 		if (i1[slot].n <= 0) return set_err(-1, "I1-type table slot %d not set", slot);
+		const int nch = term_chunks(ncell, ninst, i1[slot].n, 4);
+		int r = out.ensure((size_t)ncell * ninst * nch * GB);
+		if (r) return r;
 		I1Args a{};
 		a.src = src; a.nsrc = nsrc; a.Fhat = Fhat; a.nFh = nFh; a.fa = fa; a.fb = fb;
 		a.phi = i1[slot].phi.template as<double>(); a.psi1n = i1[slot].psi1n.p; a.nterm = i1[slot].n;
 		a.ninst = ninst;
 		for (int i = 0; i < ninst; i++) { a.instSrc[i] = instSrc[i]; a.instFh[i] = instFh[i]; }
-		a.out = out; a.tw256 = g_tw256;
-		int r = xs_for(ncell * ninst, &a.xs);
+		a.nchunk = nch;
+		a.out = out.p; a.tw256 = g_tw256;
+		r = xs_for(ncell * ninst * nch, &a.xs);
 		if (r) return r;
-		hipLaunchKernelGGL((k_i1acc<R, N, T, XG>), dim3(ncell * ninst), dim3(T), 0, stream, a);
+		hipLaunchKernelGGL((k_i1acc<R, N, T, XG>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
+		*nchunk = nch;
 		return 0;
 	}
 
-	int launch_i2(const double2* src, int nsrc, int ninst, const int* instF, const int* instG, void* out, int ncell)
+	int launch_i2(const double2* src, int nsrc, int ninst, const int* instF, const int* instG, DevBuf& out, int ncell,
+		int* nchunk)
 	{
 		if (n2 <= 0) return set_err(-1, "I2 table not set");
+		const int nch = term_chunks(ncell, ninst, n2, 2);
+		int r = out.ensure((size_t)ncell * ninst * nch * GB);
+		if (r) return r;
 		I2Args a{};
 		a.src = src; a.nsrc = nsrc; a.psi1 = i2p1.template as<double>(); a.psi2 = i2p2.template as<double>();
 		a.nterm = n2; a.ninst = ninst;
 		for (int i = 0; i < ninst; i++) { a.instF[i] = instF[i]; a.instG[i] = instG[i]; }
-		a.out = out; a.tw256 = g_tw256;
-		int r = xs_for(ncell * ninst, &a.xs);
+		a.nchunk = nch;
+		a.out = out.p; a.tw256 = g_tw256;
+		r = xs_for(ncell * ninst * nch, &a.xs);
 		if (r) return r;
-		hipLaunchKernelGGL((k_i2acc<R, N, T, XG>), dim3(ncell * ninst), dim3(T), 0, stream, a);
+		hipLaunchKernelGGL((k_i2acc<R, N, T, XG>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
+		*nchunk = nch;
 		return 0;
 	}
 
@@ -506,24 +527,22 @@ struct EngineT final : public qbf_engine {
 		const bool cc = mode & QBF_CC, cd = mode & QBF_CD;
 		const long long c4 = 4LL * PP;
 		int r = What.ensure((size_t)ncell * 4 * GB);
-		if (!r && cc) r = Ahat.ensure((size_t)ncell * 3 * GB);
-		if (!r && cd) r = Bhat.ensure((size_t)ncell * 4 * GB);
-		if (!r && cd) r = Sbuf.ensure((size_t)ncell * 4 * GB);
 		if (r) return r;
+		int nchA = 1, nchB = 1, nchS = 1;
 		const int all4[4] = {0, 1, 2, 3}, zero4[4] = {0, 0, 0, 0};
 		r = launch_padfft(W, 4, nullptr, 0, 0, 1, 4, all4, What.p, ncell);
 		if (r) return r;
 		if (cc) {
 			// -2 (2 w_tr - 1) ... : Fhat = FFT2(pad(-4 W0 + 2 delta)) = -4 What_0 + 2 (src/collision.c:220-227)
 			const int g3[3] = {1, 2, 3};
-			r = launch_i1(QBF_TAB_I1, W, 4, What.p, 4, -4.0, 2.0, 3, g3, zero4, Ahat.p, ncell);
+			r = launch_i1(QBF_TAB_I1, W, 4, What.p, 4, -4.0, 2.0, 3, g3, zero4, Ahat, ncell, &nchA);
 			if (r) return r;
 		}
 		int nchunk = 1;
 		if (cd) {
 			// I4 with Fhat = FFT2(pad(2 W0 - delta)) = 2 What_0 - 1 (src/collision.c:127-134)
-			r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat.p, ncell);
-			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf.p, ncell);
+			r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat, ncell, &nchB);
+			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf, ncell, &nchS);
 			if (r) return r;
 			pairUsed = 0;
 			pairRatio = -1.0;
@@ -552,33 +571,37 @@ struct EngineT final : public qbf_engine {
 		std::vector<CTerm> tl[4];
 		if (cc) {
 			const C* A = Ahat.template as<C>();
-			const long long a3 = 3LL * PP;
-			auto As = [&](int s) { return (const void*)(A + (size_t)s * PP); };
+			const long long a3 = 3LL * nchA * PP;
+			auto As = [&](int s) { return (const void*)(A + (size_t)s * nchA * PP); };
 			// Cc_1 = I1(f,W2,W3) - I1(f,W3,W2), cyclic (src/collision.c:231-253); Ahat slot s <-> g = s+1
-			tl[1].push_back(term(Wslot(3), c4, As(1), a3, p4));
-			tl[1].push_back(term(Wslot(2), c4, As(2), a3, -p4));
-			tl[2].push_back(term(Wslot(1), c4, As(2), a3, p4));
-			tl[2].push_back(term(Wslot(3), c4, As(0), a3, -p4));
-			tl[3].push_back(term(Wslot(2), c4, As(0), a3, p4));
-			tl[3].push_back(term(Wslot(1), c4, As(1), a3, -p4));
+			// (each slot: nchA partial sums over term ranges)
+			tl[1].push_back(term(Wslot(3), c4, As(1), a3, p4, 1, 0, nchA, PP));
+			tl[1].push_back(term(Wslot(2), c4, As(2), a3, -p4, 1, 0, nchA, PP));
+			tl[2].push_back(term(Wslot(1), c4, As(2), a3, p4, 1, 0, nchA, PP));
+			tl[2].push_back(term(Wslot(3), c4, As(0), a3, -p4, 1, 0, nchA, PP));
+			tl[3].push_back(term(Wslot(2), c4, As(0), a3, p4, 1, 0, nchA, PP));
+			tl[3].push_back(term(Wslot(1), c4, As(1), a3, -p4, 1, 0, nchA, PP));
 		}
 		if (cd) {
 			const C* B = Bhat.template as<C>();
 			const C* S = Sbuf.template as<C>();
 			const C* A3 = acc3.template as<C>();
-			auto Bs = [&](int s) { return (const void*)(B + (size_t)s * PP); };
+			const long long b4 = 4LL * nchB * PP, s4 = 4LL * nchS * PP;
+			auto Bs = [&](int s) { return (const void*)(B + (size_t)s * nchB * PP); };
 			for (int j = 0; j < 4; j++) {
 				// I4 terms (src/collision.c:137-167)
-				tl[j].push_back(term(Wslot(0), c4, Bs(j), c4, eta2[j] * p4));
-				tl[j].push_back(term(Wslot(j), c4, Bs(0), c4, 2 * p4));
+				tl[j].push_back(term(Wslot(0), c4, Bs(j), b4, eta2[j] * p4, 1, 0, nchB, PP));
+				tl[j].push_back(term(Wslot(j), c4, Bs(0), b4, 2 * p4, 1, 0, nchB, PP));
 				if (j == 0)
-					for (int jj = 0; jj < 4; jj++) tl[j].push_back(term(Wslot(jj), c4, Bs(jj), c4, -2 * p4));
+					for (int jj = 0; jj < 4; jj++)
+						tl[j].push_back(term(Wslot(jj), c4, Bs(jj), b4, -2 * p4, 1, 0, nchB, PP));
 				// I3 terms: -eta2_j / P^4 * sum_chunks acc_j (src/collision.c:110-121)
 				tl[j].push_back(term(nullptr, 0, A3 + (size_t)j * nchunk * PP, 4LL * nchunk * PP, -eta2[j] * p4,
 					1, 0, nchunk, PP));
 				// I2 terms: eta2_i / P^2 * (delta_j0 - What_j) .* S_i (src/collision.c:82-106)
 				for (int i = 0; i < 4; i++)
-					tl[j].push_back(term(Wslot(j), c4, S + (size_t)i * PP, c4, eta2[i] * p2, -1.0, j == 0 ? 1.0 : 0.0));
+					tl[j].push_back(term(Wslot(j), c4, S + (size_t)i * nchS * PP, s4, eta2[i] * p2, -1.0,
+						j == 0 ? 1.0 : 0.0, nchS, PP));
 			}
 		}
 		return launch_combine(tl, 4, all4, ncell, out, base, Wcb, Bext, cf);
@@ -610,16 +633,16 @@ struct EngineT final : public qbf_engine {
 	{
 		int r = stage3(f, g, h);
 		if (!r) r = What.ensure(2 * GB);
-		if (!r) r = Ahat.ensure(GB);
 		if (r) return r;
 		const double2* s = src3.template as<double2>();
 		const int fh[2] = {0, 2}, one[1] = {1}, z[1] = {0};
+		int nch = 1;
 		r = launch_padfft(s, 3, nullptr, 0, 0, 1, 2, fh, What.p, 1);
-		if (!r) r = launch_i1(slot, s, 3, What.p, 2, 1.0, 0.0, 1, one, z, Ahat.p, 1);
+		if (!r) r = launch_i1(slot, s, 3, What.p, 2, 1.0, 0.0, 1, one, z, Ahat, 1, &nch);
 		if (r) return r;
 		const double p4 = 1.0 / ((double)P * P * P * P);
 		std::vector<CTerm> tl[4];
-		tl[0].push_back(term(What.template as<C>() + PP, 0, Ahat.p, 0, p4));
+		tl[0].push_back(term(What.template as<C>() + PP, 0, Ahat.p, 0, p4, 1, 0, nch, PP));
 		const int c0[1] = {0};
 		return launch_combine(tl, 1, c0, 1, out, nullptr, nullptr, nullptr, 1.0);
 	}
@@ -628,16 +651,16 @@ struct EngineT final : public qbf_engine {
 	{
 		int r = stage3(f, g, h);
 		if (!r) r = What.ensure(GB);
-		if (!r) r = Sbuf.ensure(GB);
 		if (r) return r;
 		const double2* s = src3.template as<double2>();
 		const int hh[1] = {2}, f0[1] = {0}, g1[1] = {1};
+		int nch = 1;
 		r = launch_padfft(s, 3, nullptr, 0, 0, 1, 1, hh, What.p, 1);
-		if (!r) r = launch_i2(s, 3, 1, f0, g1, Sbuf.p, 1);
+		if (!r) r = launch_i2(s, 3, 1, f0, g1, Sbuf, 1, &nch);
 		if (r) return r;
 		const double p2 = 1.0 / ((double)P * P);
 		std::vector<CTerm> tl[4];
-		tl[0].push_back(term(What.p, 0, Sbuf.p, 0, p2));
+		tl[0].push_back(term(What.p, 0, Sbuf.p, 0, p2, 1, 0, nch, PP));
 		const int c0[1] = {0};
 		return launch_combine(tl, 1, c0, 1, out, nullptr, nullptr, nullptr, 1.0);
 	}
@@ -770,10 +793,11 @@ int make_engine(int maxCells, qbf_engine** out)
 	e->precision = std::is_same<R, float>::value ? QBF_FP32 : QBF_FP64;
 	e->maxCells = maxCells;
 	if (std::is_same<R, float>::value) {
-		// the paired path deviates from the complex one by about 0.15-0.8 times the measured ratio;
-		// below 1e-5 that stays under the FP32 path's own deviation from FP64 (2e-5 at N = 64)
+		// the paired path deviates from the complex one by 0.15 (Nyquist content of smooth states) to
+		// 0.8 (grid-scale noise) times the measured ratio; below 2e-5 that stays under the FP32 path's
+		// own deviation from FP64 (2.1e-5 L1 at N = 64, J = 32, M = 16)
 		e->pairMode = 1;
-		e->pairTol = 1e-5;
+		e->pairTol = 2e-5;
 	}
 	int r = e->init();
 	if (r) {

@@ -123,8 +123,11 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int lb = xcd_block();
-	const int inst = lb % a.ninst;
-	const int cell = lb / a.ninst;
+	const int chunk = lb % a.nchunk;
+	const int inst = (lb / a.nchunk) % a.ninst;
+	const int cell = lb / (a.nchunk * a.ninst);
+	const int qa = (int)(((long long)chunk * a.nterm) / a.nchunk);
+	const int qb = (int)(((long long)(chunk + 1) * a.nterm) / a.nchunk);
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instSrc[inst]) * (N * N);
 	const cx<R>* Fh = (const cx<R>*)a.Fhat + ((size_t)cell * a.nFh + a.instFh[inst]) * (P * P);
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 #pragma unroll
 	for (int s = 0; s < E; s++) acc[s] = czero<R>();
 	__syncthreads();
-	for (int q = 0; q < a.nterm; q++) {
+	for (int q = qa; q < qb; q++) {
 		padfft_reg<R, N, GE>(v, g, a.phi + (size_t)q * (N * N), 1, X, tw, line, p);
 #pragma unroll
 		for (int u = 0; u < E; u++) {
@@ -152,7 +155,7 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 		}
 	}
 	fft2_reg<R, GE>(acc, X, tw, line, p);
-	store_native<R, GE>((cx<R>*)a.out + ((size_t)cell * a.ninst + inst) * (P * P), acc);
+	store_native<R, GE>((cx<R>*)a.out + (((size_t)cell * a.ninst + inst) * a.nchunk + chunk) * (P * P), acc);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -166,8 +169,11 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int lb = xcd_block();
-	const int inst = lb % a.ninst;
-	const int cell = lb / a.ninst;
+	const int chunk = lb % a.nchunk;
+	const int inst = (lb / a.nchunk) % a.ninst;
+	const int cell = lb / (a.nchunk * a.ninst);
+	const int qa = (int)(((long long)chunk * a.nterm) / a.nchunk);
+	const int qb = (int)(((long long)(chunk + 1) * a.nterm) / a.nchunk);
 	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
 	const double2* f = a.src + ((size_t)cell * a.nsrc + a.instF[inst]) * (N * N);
 	const double2* g = a.src + ((size_t)cell * a.nsrc + a.instG[inst]) * (N * N);
@@ -176,7 +182,7 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 #pragma unroll
 	for (int u = 0; u < E; u++) acc[u] = czero<R>();
 	__syncthreads();
-	for (int q = 0; q < a.nterm; q++) {
+	for (int q = qa; q < qb; q++) {
 		padfft_reg<R, N, GE>(x1, f, a.psi1 + (size_t)q * (N * N), 1, X, tw, line, p);
 		padfft_reg<R, N, GE>(v, g, a.psi2 + (size_t)q * (N * N), 1, X, tw, line, p);
 #pragma unroll
@@ -185,7 +191,7 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 			acc[u] = cadd<R>(acc[u], t);
 		}
 	}
-	store_native<R, GE>((cx<R>*)a.out + ((size_t)cell * a.ninst + inst) * (P * P), acc);
+	store_native<R, GE>((cx<R>*)a.out + (((size_t)cell * a.ninst + inst) * a.nchunk + chunk) * (P * P), acc);
 }
 
 // ---------------------------------------------------------------------------------------------

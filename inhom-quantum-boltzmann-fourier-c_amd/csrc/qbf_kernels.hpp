@@ -37,7 +37,8 @@ struct I1Args {
 	int ninst;
 	int instSrc[8];
 	int instFh[8];
-	void* out;            // [ncell][ninst][P*P] R_b native
+	int nchunk;           // terms split into nchunk ranges (partial sums, added by k_combine)
+	void* out;            // [ncell][ninst][nchunk][P*P] R_b native
 	const double2* tw256;
 };
 
@@ -52,7 +53,8 @@ struct I2Args {
 	int ninst;
 	int instF[8];
 	int instG[8];
-	void* out;            // [ncell][ninst][P*P] R_b native
+	int nchunk;           // terms split into nchunk ranges (partial sums, added by k_combine)
+	void* out;            // [ncell][ninst][nchunk][P*P] R_b native
 	const double2* tw256;
 };
 
