@@ -309,16 +309,17 @@ __device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
 }
 
 // LDS exchanges.  The grid is stored row-major with pitch P + G and a skew of SKEW elements
-// per block of E rows.  One access phase of the gfx950 LDS serves 512 bytes: LANES lanes
-// of one element each, conflict-free when their addresses are distinct modulo LANES elements
-// (measured: lane-linear 8-byte accesses run without bank conflicts, stride-2 accesses with
-// 2-way conflicts).  Writes / reads of rows (R_i) hit LANES/G consecutive rows times G
-// consecutive columns: distinct because the pitch is G modulo LANES.  Writes / reads of
-// columns (C_b) hit LANES/G consecutive columns in G row blocks E rows apart: E*pitch is a
-// multiple of LANES, and the per-block skew LANES/G separates the blocks.
+// per block of E rows.  The LDS has 64 banks of 4 bytes: one access phase serves 256 bytes,
+// LANES consecutive lanes of one element each, conflict-free when their addresses are distinct
+// modulo LANES elements.  (A 512-byte phase model, which lane-linear and stride-2 probes do not
+// tell apart from this one, doubled the skew and tripled SQ_LDS_BANK_CONFLICT of k_i3acc.)
+// Writes / reads of rows (R_i) hit LANES/G consecutive rows times G consecutive columns:
+// distinct because the pitch is G modulo LANES.  Writes / reads of columns (C_b) hit LANES/G
+// consecutive columns in G row blocks E rows apart: E*pitch is a multiple of LANES, and the
+// per-block skew LANES/G separates the blocks.
 template <typename R, class GE>
 struct Xpose {
-	static constexpr int LANES = 512 / (int)sizeof(cx<R>);
+	static constexpr int LANES = 256 / (int)sizeof(cx<R>);
 	static constexpr int SKEW = LANES / GE::G;
 	static constexpr int SIZE = GE::P * GE::PITCH + GE::G * SKEW;
 	cx<R>* g0;
