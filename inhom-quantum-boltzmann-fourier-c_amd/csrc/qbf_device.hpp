@@ -227,6 +227,13 @@ __device__ __forceinline__ cx<R> pin(cx<R> w)
 	return w;
 }
 
+// Twiddle tables of a P-point line FFT over T threads (G = T / P lanes per line): w^j (P entries),
+// the per-lane table (P entries) and the cross-lane stage twiddles (G entries).
+template <int P, int T>
+struct TwTab {
+	static constexpr int SIZE = 2 * P + T / P;
+};
+
 // Layout of the per-lane twiddle table w^(p u): float grids interleave it by lane, [u][p], so
 // that the G distinct addresses of one read fall into distinct banks; double grids keep it
 // [p][u] (their N = 64 kernels are register-bound, and the interleaved addressing costs them
@@ -259,8 +266,17 @@ __device__ __forceinline__ void line_fwd(cx<R>* v, int p, const cx<R>* twP)
 					const cx<R> pv = shfl_xor_c<R>(v[u], h);
 					v[u] = mk<R>(pv.x + sgn * v[u].x, pv.y + sgn * v[u].y);
 				}
+			} else if (h == 2) {               // twiddle w^((p & 1) P / 4) = 1 or i
+				// the hi lanes with odd p multiply by conj(i) = -i: a swap and a sign (two selects)
+				const bool q = hi && (p & 1);
+#pragma unroll
+				for (int u = 0; u < E; u++) {
+					const cx<R> pv = shfl_xor_c<R>(v[u], h);
+					const cx<R> t = mk<R>(pv.x + sgn * v[u].x, pv.y + sgn * v[u].y);
+					v[u] = mk<R>(q ? t.y : t.x, q ? -t.x : t.y);
+				}
 			} else {
-				const cx<R> w = pin<R>(twP[(p & (h - 1)) * (P / (2 * h)) + opaque_zero()]);
+				const cx<R> w = pin<R>(twP[2 * P + h + (p & (h - 1)) + opaque_zero()]);
 				const cx<R> wl = hi ? mk<R>(w.x, -w.y) : mk<R>(R(1), R(0));
 #pragma unroll
 				for (int u = 0; u < E; u++) {
@@ -290,8 +306,17 @@ __device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
 					const cx<R> pv = shfl_xor_c<R>(v[u], h);
 					v[u] = mk<R>(pv.x + sgn * v[u].x, pv.y + sgn * v[u].y);
 				}
+			} else if (h == 2) {               // twiddle w^((p & 1) P / 4) = 1 or i
+				// the hi lanes with odd p pre-multiply by i: a swap and a sign (two selects)
+				const bool q = hi && (p & 1);
+#pragma unroll
+				for (int u = 0; u < E; u++) {
+					const cx<R> sv = mk<R>(q ? -v[u].y : v[u].x, q ? v[u].x : v[u].y);
+					const cx<R> pv = shfl_xor_c<R>(sv, h);
+					v[u] = mk<R>(pv.x + sgn * sv.x, pv.y + sgn * sv.y);
+				}
 			} else {
-				const cx<R> w = pin<R>(twP[(p & (h - 1)) * (P / (2 * h)) + opaque_zero()]);
+				const cx<R> w = pin<R>(twP[2 * P + h + (p & (h - 1)) + opaque_zero()]);
 				const cx<R> wl = hi ? w : mk<R>(R(1), R(0));
 #pragma unroll
 				for (int u = 0; u < E; u++) {
@@ -309,17 +334,17 @@ __device__ __forceinline__ void line_bwd(cx<R>* v, int p, const cx<R>* twP)
 }
 
 // LDS exchanges.  The grid is stored row-major with pitch P + G and a skew of SKEW elements
-// per block of E rows.  The LDS has 64 banks of 4 bytes: one access phase serves 256 bytes,
-// LANES consecutive lanes of one element each, conflict-free when their addresses are distinct
-// modulo LANES elements.  (A 512-byte phase model, which lane-linear and stride-2 probes do not
-// tell apart from this one, doubled the skew and tripled SQ_LDS_BANK_CONFLICT of k_i3acc.)
-// Writes / reads of rows (R_i) hit LANES/G consecutive rows times G consecutive columns:
-// distinct because the pitch is G modulo LANES.  Writes / reads of columns (C_b) hit LANES/G
-// consecutive columns in G row blocks E rows apart: E*pitch is a multiple of LANES, and the
-// per-block skew LANES/G separates the blocks.
+// per block of E rows.  An 8-byte LDS access of a wave is served 128 bytes per phase: LANES
+// consecutive lanes of one element each, conflict-free when their addresses are distinct modulo
+// LANES elements.  (Measured with SQ_LDS_BANK_CONFLICT of k_i3acc for the column skews of the
+// 512-, 256- and 128-byte phase models: 1.31e7, 4.72e6 and 5.2e5 cycles; lane-linear and stride-2
+// probes do not tell the models apart.)  Writes / reads of rows (R_i) hit LANES/G consecutive
+// rows times G consecutive columns: distinct because the pitch is G modulo LANES.  Writes /
+// reads of columns (C_b) hit LANES/G consecutive columns in G row blocks E rows apart: E*pitch
+// is a multiple of LANES, and the per-block skew LANES/G separates the blocks.
 template <typename R, class GE>
 struct Xpose {
-	static constexpr int LANES = 256 / (int)sizeof(cx<R>);
+	static constexpr int LANES = 128 / (int)sizeof(cx<R>);
 	static constexpr int SKEW = LANES / GE::G;
 	static constexpr int SIZE = GE::P * GE::PITCH + GE::G * SKEW;
 	cx<R>* g0;
@@ -348,6 +373,8 @@ struct Xpose {
 #pragma unroll
 		for (int u = 0; u < GE::E; u++) g[a(r0 + u, line)] = v[u];
 	}
+	// one element of row `row`
+	__device__ __forceinline__ cx<R> get(int row, int col) const { return gp()[a(row, col)]; }
 	// LDS -> R_i (line = row)
 	__device__ __forceinline__ void get_Ri(cx<R>* v, int line, int p) const
 	{
@@ -358,14 +385,25 @@ struct Xpose {
 };
 
 // Fill the LDS twiddle tables from a global table of (cos, sin)(2 pi j / 256) in double:
-// twP[j] = w^j for j < P, and the per-lane table w^(p*u) at twP[P + tw_lane(p) + tw_step * u],
-// w = exp(2 pi i / P).
+// twP[j] = w^j for j < P, the per-lane table w^(p*u) at twP[P + tw_lane(p) + tw_step * u], and the
+// twiddles of the cross-lane stages at twP[2P + h + m] = w^(m P / 2h), m < h < G, kept apart so
+// that the lanes of one stage read consecutive entries; w = exp(2 pi i / P).
 template <typename R, int P, int T>
 __device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
 {
 	constexpr int G = T / P, E = P / G;
-	for (int j = threadIdx.x; j < 2 * P; j += T) {
-		const int k = j < P ? j : (sizeof(R) == 4 ? ((j - P) % G) * ((j - P) / G) : ((j - P) / E) * ((j - P) % E));
+	for (int j = threadIdx.x; j < TwTab<P, T>::SIZE; j += T) {
+		int k;
+		if (j < P) {
+			k = j;
+		} else if (j < 2 * P) {
+			k = sizeof(R) == 4 ? ((j - P) % G) * ((j - P) / G) : ((j - P) / E) * ((j - P) % E);
+		} else {
+			const int i = j - 2 * P;
+			int h = 1;
+			while (2 * h <= i) h *= 2;
+			k = i == 0 ? 0 : (i - h) * (P / (2 * h));
+		}
 		const double2 w = tw256[k * (256 / P)];
 		twP[j] = mk<R>((R)w.x, (R)w.y);
 	}

@@ -320,7 +320,8 @@ struct EngineT final : public qbf_engine {
 		a.src = src; a.nsrc = nsrc; a.pre = pre; a.preKind = preKind; a.q0 = q0; a.nq = nq; a.nslot = nslot;
 		for (int i = 0; i < nslot; i++) a.srcOfSlot[i] = srcOfSlot[i];
 		a.out = out; a.tw256 = g_tw256;
-		const int nb = ncell * nq * nslot;
+		a.ngrid = ncell * nq * nslot;
+		const int nb = (a.ngrid + 1) / 2;
 		int r = xs_for(nb, &a.xs);
 		if (r) return r;
 		hipLaunchKernelGGL((k_padfft<R, N, T, XG>), dim3(nb), dim3(T), 0, stream, a);

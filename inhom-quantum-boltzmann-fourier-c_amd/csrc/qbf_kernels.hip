@@ -22,25 +22,37 @@ __device__ __forceinline__ int unpad(int x)
 
 // ---------------------------------------------------------------------------------------------
 // pruned FFT2 of pad(pre .* src): C_i load -> column FFT -> LDS -> row FFT -> R_b (registers)
+// The padded index p + G s of element s of a lane is a padding zero exactly when
+// E/4 <= s < 3E/4 (P = 2N = G E, lane p < G), the same for every lane: those inputs are
+// compile-time zeros in both passes, the zero columns are neither written to nor read from LDS.
+template <int E>
+__device__ __forceinline__ constexpr bool padded_nonzero(int s) { return s < E / 4 || s >= 3 * E / 4; }
+
 template <typename R, int N, class GE>
 __device__ __forceinline__ void padfft_reg(cx<R>* v, const double2* __restrict__ src, const void* pre, int preKind,
 	const Xpose<R, GE>& X, const cx<R>* tw, int line, int p)
 {
 	constexpr int G = GE::G, E = GE::E, P = GE::P;
+	static_assert(2 * N == G * E, "padded line of 2N = G E elements");
 	const int kn = unpad<N>(line);
+	// base pointers formed here (opaque SGPRs): in a loop over terms the per-element 64-bit
+	// addresses are then not hoisted into (spilled) registers
+	const double2* sp = uni(src, 0);
+	const double* p1 = uni((const double*)pre, 0);
+	const double2* p2 = uni((const double2*)pre, 0);
 	if (kn >= 0) {
 #pragma unroll
 		for (int s = 0; s < E; s++) {
-			const int ln = unpad<N>(p + G * s);
-			if (ln >= 0) {
+			if (padded_nonzero<E>(s)) {
+				const int ln = p + G * s - (s >= E / 4 ? N : 0);
 				const int e = ln * N + kn;
-				double2 x = src[e];
+				double2 x = sp[e];
 				if (preKind == 1) {
-					const double r = ((const double*)pre)[e];
+					const double r = p1[e];
 					x.x *= r;
 					x.y *= r;
 				} else if (preKind == 2) {
-					const double2 c = ((const double2*)pre)[e];
+					const double2 c = p2[e];
 					x = make_double2(c.x * x.x - c.y * x.y, c.x * x.y + c.y * x.x);
 				}
 				v[s] = mk<R>((R)x.x, (R)x.y);
@@ -49,14 +61,12 @@ __device__ __forceinline__ void padfft_reg(cx<R>* v, const double2* __restrict__
 			}
 		}
 		line_fwd<R, P, G, E>(v, p, tw);
-	} else {
-#pragma unroll
-		for (int s = 0; s < E; s++) v[s] = czero<R>();
 	}
 	__syncthreads();
-	X.put_Cb(v, line, p);
+	if (kn >= 0) X.put_Cb(v, line, p);
 	__syncthreads();
-	X.get_Ri(v, line, p);
+#pragma unroll
+	for (int s = 0; s < E; s++) v[s] = padded_nonzero<E>(s) ? X.get(line, p + G * s) : czero<R>();
 	line_fwd<R, P, G, E>(v, p, tw);
 }
 
@@ -90,26 +100,56 @@ __device__ __forceinline__ void fft2_reg(cx<R>* v, const Xpose<R, GE>& X, const 
 template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_padfft(PadFFTArgs a)
 {
+	// Two grids per workgroup: the N non-zero columns of grid 2b on lines 0..N-1 and those of grid
+	// 2b+1 on lines N..2N-1 (every lane works in the column pass); their transposed columns fill
+	// the LDS grid, columns 0..N-1 and N..2N-1; then a row pass per grid.
 	using GE = Geo<2 * N, T>;
-	constexpr int P = GE::P, E = GE::E;
+	constexpr int P = GE::P, E = GE::E, G = GE::G;
 	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
-	__shared__ cx<R> tw[2 * P];
+	__shared__ cx<R> tw[TwTab<P, T>::SIZE];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
-	const int o = blockIdx.x;
-	const int slot = o % a.nslot;
-	const int qq = (o / a.nslot) % a.nq;
-	const int cell = o / (a.nslot * a.nq);
-	const double2* src = a.src + ((size_t)cell * a.nsrc + a.srcOfSlot[slot]) * (N * N);
-	const void* pre = nullptr;
-	if (a.preKind == 1) pre = (const double*)a.pre + (size_t)(a.q0 + qq) * (N * N);
-	if (a.preKind == 2) pre = (const double2*)a.pre + (size_t)(a.q0 + qq) * (N * N);
-	const int line = threadIdx.x / GE::G, p = threadIdx.x % GE::G;
+	const int o0 = 2 * blockIdx.x;
+	const int line = threadIdx.x / G, p = threadIdx.x % G;
+	const int o = o0 + line / N, kn = line % N;
 	Xpose<R, GE> X{xbuf};
 	cx<R> v[E];
 	__syncthreads();
-	padfft_reg<R, N, GE>(v, src, pre, a.preKind, X, tw, line, p);
-	store_native<R, GE>((cx<R>*)a.out + (size_t)o * (P * P), v);
+	if (o < a.ngrid) {
+		const int slot = o % a.nslot;
+		const int qq = (o / a.nslot) % a.nq;
+		const int cell = o / (a.nslot * a.nq);
+		const double2* src = a.src + ((size_t)cell * a.nsrc + a.srcOfSlot[slot]) * (N * N);
+		const size_t poff = (size_t)(a.q0 + qq) * (N * N);
+#pragma unroll
+		for (int s = 0; s < E; s++) {
+			if (padded_nonzero<E>(s)) {
+				const int e = (p + G * s - (s >= E / 4 ? N : 0)) * N + kn;
+				double2 x = src[e];
+				if (a.preKind == 1) {
+					const double r = ((const double*)a.pre)[poff + e];
+					x.x *= r;
+					x.y *= r;
+				} else if (a.preKind == 2) {
+					const double2 c = ((const double2*)a.pre)[poff + e];
+					x = make_double2(c.x * x.x - c.y * x.y, c.x * x.y + c.y * x.x);
+				}
+				v[s] = mk<R>((R)x.x, (R)x.y);
+			} else {
+				v[s] = czero<R>();
+			}
+		}
+		line_fwd<R, P, G, E>(v, p, tw);
+		X.put_Cb(v, line, p);
+	}
+	__syncthreads();
+	for (int g = 0; g < 2 && o0 + g < a.ngrid; g++) {
+#pragma unroll
+		for (int s = 0; s < E; s++)
+			v[s] = padded_nonzero<E>(s) ? X.get(line, g * N + p + G * s - (s >= E / 4 ? N : 0)) : czero<R>();
+		line_fwd<R, P, G, E>(v, p, tw);
+		store_native<R, GE>((cx<R>*)a.out + (size_t)(o0 + g) * (P * P), v);
+	}
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -119,7 +159,7 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
 	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
-	__shared__ cx<R> tw[2 * P];
+	__shared__ cx<R> tw[TwTab<P, T>::SIZE];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int lb = xcd_block();
@@ -165,7 +205,7 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
 	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
-	__shared__ cx<R> tw[2 * P];
+	__shared__ cx<R> tw[TwTab<P, T>::SIZE];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int lb = xcd_block();
@@ -208,7 +248,7 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E;
 	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
-	__shared__ cx<R> tw[2 * P];
+	__shared__ cx<R> tw[TwTab<P, T>::SIZE];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int lb = xcd_block();
@@ -293,7 +333,7 @@ __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 	using GE = Geo<2 * N, T>;
 	constexpr int P = GE::P, E = GE::E, G = GE::G;
 	__shared__ cx<R> lds[XG ? 1 : Xpose<R, GE>::SIZE];
-	__shared__ cx<R> tw[2 * P];
+	__shared__ cx<R> tw[TwTab<P, T>::SIZE];
 	load_twiddles<R, P, T>(tw, a.tw256);
 	cx<R>* xbuf = XG ? (cx<R>*)a.xs + (size_t)blockIdx.x * Xpose<R, GE>::SIZE : lds;
 	const int o = blockIdx.x % a.nout;
@@ -370,7 +410,7 @@ __global__ __launch_bounds__(T) void k_phys2fourier(const double* __restrict__ i
 	using GE = Geo<N, T>;
 	constexpr int E = GE::E, G = GE::G;
 	__shared__ cx<double> lds[Xpose<double, GE>::SIZE];
-	__shared__ cx<double> tw[2 * N];
+	__shared__ cx<double> tw[TwTab<N, T>::SIZE];
 	load_twiddles<double, N, T>(tw, tw256);
 	const int gi = blockIdx.x;
 	if (gi >= ngrid) return;
@@ -396,7 +436,7 @@ __global__ __launch_bounds__(T) void k_fourier2phys(const double2* __restrict__ 
 	using GE = Geo<N, T>;
 	constexpr int E = GE::E, G = GE::G;
 	__shared__ cx<double> lds[Xpose<double, GE>::SIZE];
-	__shared__ cx<double> tw[2 * N];
+	__shared__ cx<double> tw[TwTab<N, T>::SIZE];
 	load_twiddles<double, N, T>(tw, tw256);
 	const int gi = blockIdx.x;
 	if (gi >= ngrid) return;

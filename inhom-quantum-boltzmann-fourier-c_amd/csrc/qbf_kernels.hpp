@@ -18,6 +18,7 @@ struct PadFFTArgs {
 	int q0, nq;
 	int nslot;
 	int srcOfSlot[4];
+	int ngrid;            // ncell * nq * nslot grids, two per workgroup
 	void* out;
 	const double2* tw256;
 };
