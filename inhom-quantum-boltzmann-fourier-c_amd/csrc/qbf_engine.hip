@@ -200,12 +200,13 @@ struct EngineT final : public qbf_engine {
 
 	struct I1Tab { int n = 0; DevBuf phi, psi1n; };
 	I1Tab i1[2];
-	int n2 = 0; DevBuf i2p1, i2p2;
+	int n2 = 0; DevBuf i2p1, i2p2, i2pc;
 	int n3 = 0; DevBuf i3p1, i3p2n;
 
 	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, terms, xscr;
 	DevBuf Wmid, WF, WF2, Wt1, Wt2, src3;
-	DevBuf Wpk, WhatP, reim;          // paired I3 path: packed spectra, their padded transforms, realness
+	DevBuf Wpk, WhatP, reim, Ipk;     // paired path: packed spectra, their padded transforms, realness,
+	                                  // packed I1-type results
 	size_t fBudget = (size_t)2 << 30;
 
 	int init()
@@ -286,6 +287,14 @@ struct EngineT final : public qbf_engine {
 		int r = upload(i2p1, p1, sizeof(double) * NN * (size_t)num);
 		if (!r) r = upload(i2p2, p2, sizeof(double) * NN * (size_t)num);
 		if (r) return r;
+		// psi1 + i psi2 for the paired I2 path (one transform yields both real factors)
+		std::vector<double> pc(2 * NN * (size_t)num);
+		for (size_t i = 0; i < NN * (size_t)num; i++) {
+			pc[2 * i] = p1[i];
+			pc[2 * i + 1] = p2[i];
+		}
+		r = upload(i2pc, pc.data(), sizeof(double) * pc.size());
+		if (r) return r;
 		n2 = (int)num;
 		return 0;
 	}
@@ -361,22 +370,30 @@ struct EngineT final : public qbf_engine {
 		return 0;
 	}
 
+	// pair: f = g for every instance and the grids are real (see k_i2acc, PAIR)
 	int launch_i2(const double2* src, int nsrc, int ninst, const int* instF, const int* instG, DevBuf& out, int ncell,
-		int* nchunk)
+		int* nchunk, bool pair = false)
 	{
 		if (n2 <= 0) return set_err(-1, "I2 table not set");
+		if (pair)
+			for (int i = 0; i < ninst; i++)
+				if (instF[i] != instG[i]) return set_err(-1, "paired I2 needs f = g");
 		const int nch = term_chunks(ncell, ninst, n2, 2);
 		int r = out.ensure((size_t)ncell * ninst * nch * GB);
 		if (r) return r;
 		I2Args a{};
 		a.src = src; a.nsrc = nsrc; a.psi1 = i2p1.template as<double>(); a.psi2 = i2p2.template as<double>();
+		a.psic = i2pc.template as<double2>();
 		a.nterm = n2; a.ninst = ninst;
 		for (int i = 0; i < ninst; i++) { a.instF[i] = instF[i]; a.instG[i] = instG[i]; }
 		a.nchunk = nch;
 		a.out = out.p; a.tw256 = g_tw256;
 		r = xs_for(ncell * ninst * nch, &a.xs);
 		if (r) return r;
-		hipLaunchKernelGGL((k_i2acc<R, N, T, XG>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
+		if (pair)
+			hipLaunchKernelGGL((k_i2acc<R, N, T, XG, true>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
+		else
+			hipLaunchKernelGGL((k_i2acc<R, N, T, XG, false>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		*nchunk = nch;
 		return 0;
@@ -458,18 +475,40 @@ struct EngineT final : public qbf_engine {
 		return 0;
 	}
 
-	// I3 part of CdInt on packed component pairs (see k_i3acc, PAIR)
-	int run_i3_pairs(const double2* W, int ncell, int* nchunk)
+	// Packed sources of the paired path, [ncell][3]: W0 + i W1, W2 + i W3, W1 (see k_pack_pairs)
+	int pack_sources(const double2* W, int ncell)
 	{
-		int r = Wpk.ensure(sizeof(double2) * 2 * NN * (size_t)ncell);
-		if (!r) r = WhatP.ensure((size_t)ncell * 2 * GB);
+		int r = Wpk.ensure(sizeof(double2) * 3 * NN * (size_t)ncell);
 		if (r) return r;
-		const long long n = 2LL * NN * ncell;
+		const long long n = 3LL * NN * ncell;
 		hipLaunchKernelGGL(k_pack_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, W,
 			Wpk.template as<double2>(), NN, n);
 		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	// Real parts and imaginary parts of packed I1-type results [ncell][npk][nch] into the slots
+	// dst[k][0], dst[k][1] of out [ncell][nout][nch] (imaginary part zero); -1: not used.
+	int unpack_real(const DevBuf& packed, int npk, DevBuf& out, int nout, int nch, const int (*dst)[2], int ncell)
+	{
+		int r = out.ensure((size_t)ncell * nout * nch * GB);
+		if (r) return r;
+		UnpackArgs a{};
+		a.in = packed.p; a.out = out.p; a.npk = npk; a.nout = nout; a.nch = nch;
+		for (int k = 0; k < npk; k++) { a.dst[k][0] = dst[k][0]; a.dst[k][1] = dst[k][1]; }
+		const long long n = (long long)ncell * npk * nch * PP;
+		hipLaunchKernelGGL((k_unpack_real<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, n, (long long)PP);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
+	// I3 part of CdInt on packed component pairs (see k_i3acc, PAIR); Wpk from pack_sources
+	int run_i3_pairs(int ncell, int* nchunk)
+	{
+		int r = WhatP.ensure((size_t)ncell * 2 * GB);
+		if (r) return r;
 		const int two[2] = {0, 1};
-		r = launch_padfft(Wpk.template as<double2>(), 2, nullptr, 0, 0, 1, 2, two, WhatP.p, ncell);
+		r = launch_padfft(Wpk.template as<double2>(), 3, nullptr, 0, 0, 1, 2, two, WhatP.p, ncell);
 		if (r) return r;
 		static const double eta[4] = {1, -1, -1, -1};
 		int ni[4] = {2, 2, 2, 2};
@@ -483,7 +522,7 @@ struct EngineT final : public qbf_engine {
 				iW[j][2 * pr] = eta[2 * pr];
 				iW[j][2 * pr + 1] = eta[2 * pr + 1];
 			}
-		return run_i3(Wpk.template as<double2>(), 2, 2, two, WhatP.p, 2, 4, ni, iF, iS, iH, iW, ncell, nchunk, true);
+		return run_i3(Wpk.template as<double2>(), 3, 2, two, WhatP.p, 2, 4, ni, iF, iS, iH, iW, ncell, nchunk, true);
 	}
 
 	int launch_combine(std::vector<CTerm> (&tl)[4], int nout, const int* outComp, int ncell, double2* out,
@@ -533,27 +572,50 @@ struct EngineT final : public qbf_engine {
 		const int all4[4] = {0, 1, 2, 3}, zero4[4] = {0, 0, 0, 0};
 		r = launch_padfft(W, 4, nullptr, 0, 0, 1, 4, all4, What.p, ncell);
 		if (r) return r;
+		// paired path: every I1, I2 and I3 chain of a real state is real (see k_i3acc, k_i2acc); the I1
+		// chain is complex-linear in its source, so W_a + i W_b yields out_a + i out_b
+		pairUsed = 0;
+		pairRatio = -1.0;
+		if (pairMode == 1) {
+			r = measure_realness(ncell, &pairRatio);
+			if (r) return r;
+		}
+		const bool pair = pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol);
+		if (pair) {
+			pairUsed = 1;
+			r = pack_sources(W, ncell);
+			if (r) return r;
+		}
 		if (cc) {
 			// -2 (2 w_tr - 1) ... : Fhat = FFT2(pad(-4 W0 + 2 delta)) = -4 What_0 + 2 (src/collision.c:220-227)
-			const int g3[3] = {1, 2, 3};
-			r = launch_i1(QBF_TAB_I1, W, 4, What.p, 4, -4.0, 2.0, 3, g3, zero4, Ahat, ncell, &nchA);
+			if (pair) {
+				// packed sources W2 + i W3 and W1 -> Ahat slots (1, 2) and (0, -)
+				const int srcs[2] = {1, 2}, dst[2][2] = {{1, 2}, {0, -1}};
+				r = launch_i1(QBF_TAB_I1, Wpk.template as<double2>(), 3, What.p, 4, -4.0, 2.0, 2, srcs, zero4, Ipk,
+					ncell, &nchA);
+				if (!r) r = unpack_real(Ipk, 2, Ahat, 3, nchA, dst, ncell);
+			} else {
+				const int g3[3] = {1, 2, 3};
+				r = launch_i1(QBF_TAB_I1, W, 4, What.p, 4, -4.0, 2.0, 3, g3, zero4, Ahat, ncell, &nchA);
+			}
 			if (r) return r;
 		}
 		int nchunk = 1;
 		if (cd) {
 			// I4 with Fhat = FFT2(pad(2 W0 - delta)) = 2 What_0 - 1 (src/collision.c:127-134)
-			r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat, ncell, &nchB);
-			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf, ncell, &nchS);
-			if (r) return r;
-			pairUsed = 0;
-			pairRatio = -1.0;
-			if (pairMode == 1) {
-				r = measure_realness(ncell, &pairRatio);
-				if (r) return r;
+			if (pair) {
+				// packed sources W0 + i W1 and W2 + i W3 -> Bhat slots (0, 1) and (2, 3)
+				const int srcs[2] = {0, 1}, dst[2][2] = {{0, 1}, {2, 3}};
+				r = launch_i1(QBF_TAB_I4, Wpk.template as<double2>(), 3, What.p, 4, 2.0, -1.0, 2, srcs, zero4, Ipk,
+					ncell, &nchB);
+				if (!r) r = unpack_real(Ipk, 2, Bhat, 4, nchB, dst, ncell);
+			} else {
+				r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat, ncell, &nchB);
 			}
-			if (pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol)) {
-				pairUsed = 1;
-				r = run_i3_pairs(W, ncell, &nchunk);
+			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf, ncell, &nchS, pair);
+			if (r) return r;
+			if (pair) {
+				r = run_i3_pairs(ncell, &nchunk);
 			} else {
 				static const double eta[4] = {1, -1, -1, -1};
 				int ni[4] = {4, 4, 4, 4};

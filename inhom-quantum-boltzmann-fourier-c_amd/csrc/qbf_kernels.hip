@@ -199,7 +199,10 @@ __global__ __launch_bounds__(T) void k_i1acc(I1Args a)
 }
 
 // ---------------------------------------------------------------------------------------------
-template <typename R, int N, int T, bool XG>
+// PAIR (f = g, real grids): FFT2(pad(psi1 f)) and FFT2(pad(psi2 f)) are the real and imaginary
+// parts of FFT2(pad((psi1 + i psi2) f)) when both are real (psi1, psi2 even, f the spectrum of a
+// real state with empty Nyquist lines), so one transform per term gives the product re * im.
+template <typename R, int N, int T, bool XG, bool PAIR>
 __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 {
 	using GE = Geo<2 * N, T>;
@@ -223,12 +226,18 @@ __global__ __launch_bounds__(T) void k_i2acc(I2Args a)
 	for (int u = 0; u < E; u++) acc[u] = czero<R>();
 	__syncthreads();
 	for (int q = qa; q < qb; q++) {
-		padfft_reg<R, N, GE>(x1, f, a.psi1 + (size_t)q * (N * N), 1, X, tw, line, p);
-		padfft_reg<R, N, GE>(v, g, a.psi2 + (size_t)q * (N * N), 1, X, tw, line, p);
+		if constexpr (PAIR) {
+			padfft_reg<R, N, GE>(v, f, a.psic + (size_t)q * (N * N), 2, X, tw, line, p);
 #pragma unroll
-		for (int u = 0; u < E; u++) {
-			const cx<R> t = cmul<R>(x1[u], v[u]);
-			acc[u] = cadd<R>(acc[u], t);
+			for (int u = 0; u < E; u++) acc[u].x += v[u].x * v[u].y;
+		} else {
+			padfft_reg<R, N, GE>(x1, f, a.psi1 + (size_t)q * (N * N), 1, X, tw, line, p);
+			padfft_reg<R, N, GE>(v, g, a.psi2 + (size_t)q * (N * N), 1, X, tw, line, p);
+#pragma unroll
+			for (int u = 0; u < E; u++) {
+				const cx<R> t = cmul<R>(x1[u], v[u]);
+				acc[u] = cadd<R>(acc[u], t);
+			}
 		}
 	}
 	store_native<R, GE>((cx<R>*)a.out + (((size_t)cell * a.ninst + inst) * a.nchunk + chunk) * (P * P), acc);
@@ -817,19 +826,42 @@ __global__ __launch_bounds__(256) void k_cell_diag(const double* __restrict__ W,
 }
 
 // ---------------------------------------------------------------------------------------------
-// Packed component pairs for the paired I3 path: Wp[cell][p] = W[cell][2p] + i W[cell][2p+1]
-// (n = ncell * 2 * NN elements).  The padded transform of Wp (and of psi1 .* Wp) is the pair of
-// transforms of the two components in its real and imaginary parts when both are real.
+// Packed component pairs for the paired path: Wp[cell] = {W0 + i W1, W2 + i W3, W1} (n = ncell * 3 * NN
+// elements).  The padded transform of a packed slot (and of psi .* it) is the pair of transforms of
+// its two components in its real and imaginary parts when both are real.
 __global__ __launch_bounds__(256) void k_pack_pairs(const double2* __restrict__ W, double2* __restrict__ Wp, int NN,
 	long long n)
 {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const long long cell = i / (2 * NN);
-	const int r = (int)(i % (2 * NN));
+	const long long cell = i / (3 * NN);
+	const int r = (int)(i % (3 * NN));
 	const int pr = r / NN, e = r % NN;
+	if (pr == 2) {
+		Wp[i] = W[(cell * 4 + 1) * NN + e];
+		return;
+	}
 	const double2 x = W[(cell * 4 + 2 * pr) * NN + e], y = W[(cell * 4 + 2 * pr + 1) * NN + e];
 	Wp[i] = make_double2(x.x - y.y, x.y + y.x);
+}
+
+// Unpacking of packed I1-type results: in [ncell][npk][nch][PP] -> out [ncell][nout][nch][PP],
+// the real part of packed slot k into slot dst[k][0], the imaginary part into dst[k][1] (each with
+// imaginary part zero; -1: dropped).  n = ncell * npk * nch * PP.
+template <typename R>
+__global__ __launch_bounds__(256) void k_unpack_real(UnpackArgs a, long long n, long long PP)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const long long e = i % PP, t = i / PP;
+	const int ch = (int)(t % a.nch), k = (int)((t / a.nch) % a.npk);
+	const long long cell = t / ((long long)a.nch * a.npk);
+	const cx<R> z = ((const cx<R>*)a.in)[i];
+	for (int c = 0; c < 2; c++) {
+		const int d = a.dst[k][c];
+		if (d >= 0)
+			((cx<R>*)a.out)[((cell * a.nout + d) * a.nch + ch) * PP + e] = mk<R>(c ? z.y : z.x, R(0));
+	}
 }
 
 // Largest |real part| and |imaginary part| of n complex values: out[block] = (max |re|, max |im|)

@@ -23,6 +23,14 @@ struct PadFFTArgs {
 	const double2* tw256;
 };
 
+// Unpacking of packed real results (paired path)
+struct UnpackArgs {
+	const void* in;
+	void* out;
+	int npk, nout, nch;
+	int dst[4][2];
+};
+
 // I1-type accumulation (I1 for Cc, I1 with FourierI4 tables for I4):
 //   out[cell][inst] = FFT2( sum_g psi1_g .* IFFT2( (fa*Fhat + fb) .* FFT2(pad(phi_g .* src)) ) )
 struct I1Args {
@@ -50,6 +58,7 @@ struct I2Args {
 	int nsrc;
 	const double* psi1;   // [nterm][N*N]
 	const double* psi2;   // [nterm][N*N]
+	const double2* psic;  // [nterm][N*N] psi1 + i psi2 (paired path)
 	int nterm;
 	int ninst;
 	int instF[8];
