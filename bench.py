@@ -46,7 +46,7 @@ REF_FILE = os.path.join(ROOT, "tests", "golden", "periodic64_fp64_ref.npz")
 # 256 MiB, accumulator read + write 512 MiB, psi2 0.5 MiB).
 # HBM bytes per k_i3acc launch at N=64, Nx=512 (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated):
 # complex kernel (8 q x 512 cells x 16 applications) and paired kernel (16 q x 512 cells x 8)
-I3_TRAFFIC_BYTES = {False: 10.34e9, True: 5.32e9}
+I3_TRAFFIC_BYTES = {False: 10.34e9, True: 5.18e9}
 
 
 def cpu_baseline(N, J, M, cores, seed):

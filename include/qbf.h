@@ -131,10 +131,11 @@ int qbf_moment_correct(qbf_engine *e, const double *dWold, double *dWnew, int nc
  * (visualization/wignerMoments.m:15-37, entropy.m, visualizeHomBoltzmann.m:26-45,98-105). */
 int qbf_cell_diagnostics(qbf_engine *e, const double *dW, int ncell, double L, double *dOut);
 
-/* Paired I3 path (no counterpart in the reference; an option of the FP32 convolution path).  For
+/* Paired path (no counterpart in the reference; an option of the FP32 convolution path).  For
  * spectra of real states every grid that K_q = FFT2(psi2 .* IFFT2(.)) acts on is real, so one complex
- * application carries two; the imaginary parts it drops come from the Nyquist lines of the state
- * (zero padding places them without their conjugate partners).  mode 0: off; 1: on when the
+ * application carries two, and the I1 and I2 chains of two components share one transform; the
+ * imaginary parts this drops come from the Nyquist lines of the state (zero padding places them
+ * without their conjugate partners).  mode 0: off; 1: on when the
  * measured max |Im| / max |Re| of the padded physical grids of the state is at most tol; 2: always.
  * The paired result deviates from the complex one by about 0.15-0.8 times the ratio.  Defaults:
  * FP64 engines 0, FP32 engines 1 with tol = 2e-5.  qbf_real_pairs_status: whether the last collision evaluation used it, and the ratio
