@@ -125,31 +125,40 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 	for (int m = 0; m < MT; m++)
 #pragma unroll
 		for (int t = 0; t < NT; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-	for (int ky = 0; ky < 3; ky++) {
-		const int ys = (y + DIR * dil * (ky - 1) + 2 * S) & (S - 1);
-		const bf16* rowp = in + ((size_t)b * S + ys) * S * KC;
-		for (int kx = 0; kx < 3; kx++) {
-			const bf16* wt = wl + (ky * 3 + kx) * NC * KP;
-			const int sh = DIR * dil * (kx - 1) + 2 * S;
-#pragma unroll 1
-			for (int kb = 0; kb < KC / 32; kb++) {
-				bf16x8 w[NT];
+	// K loop over the 9 taps x KC/32 channel blocks, the input fragments of the next iteration
+	// loaded while the MFMAs of this one run (two iterations per loop body: the fragments ping-pong
+	// between two register sets)
+	constexpr int KB = KC / 32, NIT = 9 * KB;
+	auto load_a = [&](int it, bf16x8* a) {
+		const int tap = it / KB, kb = it % KB;
+		const int ys = (y + DIR * dil * (tap / 3 - 1) + 2 * S) & (S - 1);
+		const int sh = DIR * dil * (tap % 3 - 1) + 2 * S;
+		const bf16* rowp = in + ((size_t)b * S + ys) * S * KC + kb * 32 + kq * 8;
 #pragma unroll
-				for (int t = 0; t < NT; t++) {
-					const bf16* q = wt + (t * 16 + i16) * KP + kb * 32 + kq * 8;
-					uint2 lo = *(const uint2*)q, hi = *(const uint2*)(q + 4);
-					uint4 u = {lo.x, lo.y, hi.x, hi.y};
-					w[t] = *(const bf16x8*)&u;
-				}
+		for (int m = 0; m < MT; m++) a[m] = *(const bf16x8*)(rowp + (size_t)((m * 16 + i16 + sh) & (S - 1)) * KC);
+	};
+	bf16x8 a[MT];
+	load_a(0, a);
+#pragma unroll 2
+	for (int it = 0; it < NIT; it++) {
+		const int tap = it / KB, kb = it % KB;
+		const bf16* wt = wl + tap * NC * KP;
+		bf16x8 w[NT];
 #pragma unroll
-				for (int m = 0; m < MT; m++) {
-					const int xs = (m * 16 + i16 + sh) & (S - 1);
-					const bf16x8 a = *(const bf16x8*)(rowp + (size_t)xs * KC + kb * 32 + kq * 8);
-#pragma unroll
-					for (int t = 0; t < NT; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[t], acc[m][t], 0, 0, 0);
-				}
-			}
+		for (int t = 0; t < NT; t++) {
+			const bf16* q = wt + (t * 16 + i16) * KP + kb * 32 + kq * 8;
+			uint2 lo = *(const uint2*)q, hi = *(const uint2*)(q + 4);
+			uint4 u = {lo.x, lo.y, hi.x, hi.y};
+			w[t] = *(const bf16x8*)&u;
 		}
+		bf16x8 an[MT];
+		load_a(it + 1 < NIT ? it + 1 : it, an);
+#pragma unroll
+		for (int m = 0; m < MT; m++)
+#pragma unroll
+			for (int t = 0; t < NT; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], w[t], acc[m][t], 0, 0, 0);
+#pragma unroll
+		for (int m = 0; m < MT; m++) a[m] = an[m];
 	}
 	// lane holds D[4*kq + r][i16] of tile (m, t): pixel m*16 + 4*kq + r, channel t*16 + i16
 	float osc = oscale;
