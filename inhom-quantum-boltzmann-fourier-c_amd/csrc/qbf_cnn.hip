@@ -40,18 +40,28 @@ constexpr int CREAL = 4;        // real input / output channels (Pauli component
 __host__ __device__ constexpr int cin_of(int l) { return l == 0 ? CIO : CH; }
 __host__ __device__ constexpr int cout_of(int l) { return l == NL - 1 ? CIO : CH; }
 
+// tanh-form GELU, z * 0.5 (1 + tanh u) with u = sqrt(2/pi) (z + 0.044715 z^3), evaluated through
+// 0.5 (1 + tanh u) = 1 / (1 + exp(-2u)) with the hardware base-2 exponential and reciprocal (about
+// 1e-7 relative error, far below the bf16 rounding of the activations; exp overflow gives 0, the
+// z -> -inf limit)
+__device__ __forceinline__ float half_one_plus_tanh(float u)
+{
+	const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * u);   // exp(-2u) = 2^(-2 log2(e) u)
+	return __builtin_amdgcn_rcpf(1.0f + e);
+}
 __device__ __forceinline__ float gelu(float z)
 {
 	const float c = 0.7978845608028654f;   // sqrt(2/pi)
 	const float u = c * (z + 0.044715f * z * z * z);
-	return 0.5f * z * (1.0f + tanhf(u));
+	return z * half_one_plus_tanh(u);
 }
+// derivative: s + z s (1 - s) 2 u', s = 0.5 (1 + tanh u), u' = sqrt(2/pi) (1 + 3 * 0.044715 z^2)
 __device__ __forceinline__ float gelu_grad(float z)
 {
 	const float c = 0.7978845608028654f;
 	const float u = c * (z + 0.044715f * z * z * z);
-	const float t = tanhf(u);
-	return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
+	const float sg = half_one_plus_tanh(u);
+	return sg + 2.0f * z * sg * (1.0f - sg) * c * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
 // a[b] = max |x[b][0][.]|  (one workgroup of 256 per cell)
@@ -163,6 +173,22 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 	// lane holds D[4*kq + r][i16] of tile (m, t): pixel m*16 + 4*kq + r, channel t*16 + i16
 	float osc = oscale;
 	if (MODE == 1 && amp) osc *= amp[b] * amp[b];
+	if constexpr (MODE == 1) {
+		// the CREAL output channels of the row go through LDS (the weights are no longer needed)
+		// so that the NCHW float rows are written whole: [c][S] per wave, 16 bytes per lane
+		__syncthreads();
+		float* ys = (float*)wl + (threadIdx.x >> 6) * (CREAL * S);
+		if (i16 < CREAL)
+#pragma unroll
+			for (int m = 0; m < MT; m++)
+#pragma unroll
+				for (int r = 0; r < 4; r++) ys[i16 * S + m * 16 + kq * 4 + r] = osc * (acc[m][0][r] + bias[i16]);
+		__syncthreads();
+		static_assert(CREAL * S == 4 * 64, "one float4 per lane");
+		const int c = lane / (S / 4), x4 = (lane % (S / 4)) * 4;
+		*(float4*)(yout + ((size_t)b * CREAL + c) * (S * S) + (size_t)y * S + x4) = *(const float4*)(ys + c * S + x4);
+		return;
+	}
 #pragma unroll
 	for (int m = 0; m < MT; m++)
 #pragma unroll
