@@ -388,11 +388,12 @@ struct Xpose {
 // twP[j] = w^j for j < P, the per-lane table w^(p*u) at twP[P + tw_lane(p) + tw_step * u], and the
 // twiddles of the cross-lane stages at twP[2P + h + m] = w^(m P / 2h), m < h < G, kept apart so
 // that the lanes of one stage read consecutive entries; w = exp(2 pi i / P).
-template <typename R, int P, int T>
-__device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
+// (load_twiddles_g: G lanes per line, filled by the NT threads of the workgroup)
+template <typename R, int P, int G, int NT>
+__device__ __forceinline__ void load_twiddles_g(cx<R>* twP, const double2* tw256)
 {
-	constexpr int G = T / P, E = P / G;
-	for (int j = threadIdx.x; j < TwTab<P, T>::SIZE; j += T) {
+	constexpr int E = P / G;
+	for (int j = threadIdx.x; j < 2 * P + G; j += NT) {
 		int k;
 		if (j < P) {
 			k = j;
@@ -407,6 +408,12 @@ __device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
 		const double2 w = tw256[k * (256 / P)];
 		twP[j] = mk<R>((R)w.x, (R)w.y);
 	}
+}
+template <typename R, int P, int T>
+__device__ __forceinline__ void load_twiddles(cx<R>* twP, const double2* tw256)
+{
+	static_assert(TwTab<P, T>::SIZE == 2 * P + T / P, "table layout");
+	load_twiddles_g<R, P, T / P, T>(twP, tw256);
 }
 
 // Native (coalesced) storage of a register-resident grid.

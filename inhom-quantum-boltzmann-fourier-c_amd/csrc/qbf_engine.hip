@@ -12,7 +12,7 @@
 // one angle (src/quadrature.c:192-195, 213-234) so the J^2 terms are summed into J
 // groups with Phi_i = sum_j psiR2_ij; the final convolution and Extract of every
 // integral are linear and are hoisted out of the term sums.
-#include "qbf_kernels.hip"
+#include "qbf_big.hip"
 #include "../../include/qbf.h"
 
 #include <math.h>
@@ -60,10 +60,16 @@ template <> struct Cfg<float, 32> { static constexpr int T = 256; static constex
 #define QBF_F32_N64_THREADS 1024   // 512 also fits in registers (G = 4, E = 32)
 #endif
 template <> struct Cfg<float, 64> { static constexpr int T = QBF_F32_N64_THREADS; static constexpr bool XG = false; };
+// N = 128: the padded grids stay in global memory (qbf_big.hip: one workgroup of 1024 (FP32) or 512
+// (FP64) threads per grid, line sweeps; T must match big::TB, XG sizes the scratch per grid)
+template <> struct Cfg<double, 128> { static constexpr int T = 512; static constexpr bool XG = true; };
+template <> struct Cfg<float, 128> { static constexpr int T = 1024; static constexpr bool XG = true; };
 template <int N> struct CfgNN;
-template <> struct CfgNN<16> { static constexpr int T = 64; };
-template <> struct CfgNN<32> { static constexpr int T = 128; };
-template <> struct CfgNN<64> { static constexpr int T = 256; };
+template <> struct CfgNN<16> { static constexpr int T = 64; static constexpr bool XG = false; };
+template <> struct CfgNN<32> { static constexpr int T = 128; static constexpr bool XG = false; };
+template <> struct CfgNN<64> { static constexpr int T = 256; static constexpr bool XG = false; };
+// an FP64 128 x 128 state grid (256 KiB) exchanges through global memory as well
+template <> struct CfgNN<128> { static constexpr int T = 1024; static constexpr bool XG = true; };
 
 struct qbf_engine {
 	virtual ~qbf_engine()
@@ -194,6 +200,8 @@ struct EngineT final : public qbf_engine {
 	static constexpr int T = Cfg<R, N>::T;
 	static constexpr int TN = CfgNN<N>::T;
 	static constexpr bool XG = Cfg<R, N>::XG;
+	static constexpr bool BIG = N == 128;        // P = 256 kernels of qbf_big.hip, natural grid order
+	static_assert(!BIG || T == big::TB<R>, "launch size of the N = 128 kernels");
 	using GE = Geo<P, T>;
 	using C = cx<R>;
 	static constexpr size_t GB = sizeof(C) * PP;    // bytes of one padded grid
@@ -220,6 +228,10 @@ struct EngineT final : public qbf_engine {
 	// --------------------------------------------------------------- tables
 	static void to_Ci_native(const double* tab, R* out)
 	{
+		if constexpr (BIG) {
+			for (int i = 0; i < PP; i++) out[i] = (R)tab[i];
+			return;
+		}
 		for (int t = 0; t < T; t++) {
 			const int line = t / GE::G, p = t % GE::G;
 			for (int s = 0; s < GE::E; s++) {
@@ -316,7 +328,8 @@ struct EngineT final : public qbf_engine {
 	{
 		*xs = nullptr;
 		if (!XG) return 0;
-		int r = xscr.ensure((size_t)nblocks * Xpose<R, GE>::SIZE * sizeof(C));
+		const size_t per = BIG ? (size_t)PP : (size_t)Xpose<R, GE>::SIZE;
+		int r = xscr.ensure((size_t)nblocks * per * sizeof(C));
 		if (r) return r;
 		*xs = xscr.p;
 		return 0;
@@ -330,10 +343,13 @@ struct EngineT final : public qbf_engine {
 		for (int i = 0; i < nslot; i++) a.srcOfSlot[i] = srcOfSlot[i];
 		a.out = out; a.tw256 = g_tw256;
 		a.ngrid = ncell * nq * nslot;
-		const int nb = (a.ngrid + 1) / 2;
+		const int nb = BIG ? a.ngrid : (a.ngrid + 1) / 2;
 		int r = xs_for(nb, &a.xs);
 		if (r) return r;
-		hipLaunchKernelGGL((k_padfft<R, N, T, XG>), dim3(nb), dim3(T), 0, stream, a);
+		if constexpr (BIG)
+			hipLaunchKernelGGL((k_padfft_big<R>), dim3(nb), dim3(T), 0, stream, a);
+		else
+			hipLaunchKernelGGL((k_padfft<R, N, T, XG>), dim3(nb), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		return 0;
 	}
@@ -364,7 +380,10 @@ struct EngineT final : public qbf_engine {
 		a.out = out.p; a.tw256 = g_tw256;
 		r = xs_for(ncell * ninst * nch, &a.xs);
 		if (r) return r;
-		hipLaunchKernelGGL((k_i1acc<R, N, T, XG>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
+		if constexpr (BIG)
+			hipLaunchKernelGGL((k_i1acc_big<R>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
+		else
+			hipLaunchKernelGGL((k_i1acc<R, N, T, XG>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		*nchunk = nch;
 		return 0;
@@ -375,6 +394,7 @@ struct EngineT final : public qbf_engine {
 		int* nchunk, bool pair = false)
 	{
 		if (n2 <= 0) return set_err(-1, "I2 table not set");
+		if (BIG && pair) return set_err(-1, "no paired path at N = 128");
 		if (pair)
 			for (int i = 0; i < ninst; i++)
 				if (instF[i] != instG[i]) return set_err(-1, "paired I2 needs f = g");
@@ -390,7 +410,9 @@ struct EngineT final : public qbf_engine {
 		a.out = out.p; a.tw256 = g_tw256;
 		r = xs_for(ncell * ninst * nch, &a.xs);
 		if (r) return r;
-		if (pair)
+		if constexpr (BIG)
+			hipLaunchKernelGGL((k_i2acc_big<R>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
+		else if (pair)
 			hipLaunchKernelGGL((k_i2acc<R, N, T, XG, true>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
 		else
 			hipLaunchKernelGGL((k_i2acc<R, N, T, XG, false>), dim3(ncell * ninst * nch), dim3(T), 0, stream, a);
@@ -405,6 +427,7 @@ struct EngineT final : public qbf_engine {
 		int ncell, int* nchunkOut, bool pair = false)
 	{
 		if (n3 <= 0) return set_err(-1, "I3 table not set");
+		if (BIG && pair) return set_err(-1, "no paired path at N = 128");
 		const int nq = n3;
 		int nchunk = (1024 + ncell * nout - 1) / (ncell * nout);
 		if (nchunk > nq) nchunk = nq;
@@ -437,7 +460,9 @@ struct EngineT final : public qbf_engine {
 			if (r) return r;
 			hipEvent_t eb = nullptr, ee = nullptr;
 			if (prof && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
-			if (pair)
+			if constexpr (BIG)
+				hipLaunchKernelGGL((k_i3acc_big<R>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			else if (pair)
 				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, true>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			else
 				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, false>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
@@ -543,7 +568,10 @@ struct EngineT final : public qbf_engine {
 		a.tw256 = g_tw256;
 		r = xs_for(ncell * nout, &a.xs);
 		if (r) return r;
-		hipLaunchKernelGGL((k_combine<R, N, T, XG>), dim3(ncell * nout), dim3(T), 0, stream, a);
+		if constexpr (BIG)
+			hipLaunchKernelGGL((k_combine_big<R>), dim3(ncell * nout), dim3(T), 0, stream, a);
+		else
+			hipLaunchKernelGGL((k_combine<R, N, T, XG>), dim3(ncell * nout), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
 		// the host term table must outlive the asynchronous copy
 		HIPCHK(hipStreamSynchronize(stream));
@@ -576,11 +604,11 @@ struct EngineT final : public qbf_engine {
 		// chain is complex-linear in its source, so W_a + i W_b yields out_a + i out_b
 		pairUsed = 0;
 		pairRatio = -1.0;
-		if (pairMode == 1) {
+		if (!BIG && pairMode == 1) {
 			r = measure_realness(ncell, &pairRatio);
 			if (r) return r;
 		}
-		const bool pair = pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol);
+		const bool pair = !BIG && (pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol));
 		if (pair) {
 			pairUsed = 1;
 			r = pack_sources(W, ncell);
@@ -751,17 +779,41 @@ struct EngineT final : public qbf_engine {
 	}
 
 	// --------------------------------------------------------------- state transforms and transport
+	// N x N state transforms; with XGN the exchange goes through xscrN, at most NB_XN grids a launch
+	static constexpr bool XGN = CfgNN<N>::XG;
+	static constexpr int NB_XN = 2048;
+	DevBuf xscrN;
 	int phys2fourier(const double* in, int ngrid, double2* out) override
 	{
-		hipLaunchKernelGGL((k_phys2fourier<N, TN>), dim3(ngrid), dim3(TN), 0, stream, in, out, g_tw256, ngrid);
-		HIPCHK(hipGetLastError());
+		for (int g0 = 0; g0 < ngrid; g0 += NB_XN) {
+			const int ng = ngrid - g0 < NB_XN ? ngrid - g0 : NB_XN;
+			double2* xs = nullptr;
+			if (XGN) {
+				int r = xscrN.ensure(sizeof(double2) * (size_t)NB_XN * Xpose<double, Geo<N, TN>>::SIZE);
+				if (r) return r;
+				xs = xscrN.template as<double2>();
+			}
+			hipLaunchKernelGGL((k_phys2fourier<N, TN, XGN>), dim3(ng), dim3(TN), 0, stream, in + (size_t)g0 * NN,
+				out + (size_t)g0 * NN, g_tw256, ng, xs);
+			HIPCHK(hipGetLastError());
+		}
 		return 0;
 	}
 
 	int fourier2phys(const double2* in, int ngrid, double* out) override
 	{
-		hipLaunchKernelGGL((k_fourier2phys<N, TN>), dim3(ngrid), dim3(TN), 0, stream, in, out, g_tw256, ngrid);
-		HIPCHK(hipGetLastError());
+		for (int g0 = 0; g0 < ngrid; g0 += NB_XN) {
+			const int ng = ngrid - g0 < NB_XN ? ngrid - g0 : NB_XN;
+			double2* xs = nullptr;
+			if (XGN) {
+				int r = xscrN.ensure(sizeof(double2) * (size_t)NB_XN * Xpose<double, Geo<N, TN>>::SIZE);
+				if (r) return r;
+				xs = xscrN.template as<double2>();
+			}
+			hipLaunchKernelGGL((k_fourier2phys<N, TN, XGN>), dim3(ng), dim3(TN), 0, stream, in + (size_t)g0 * NN,
+				out + (size_t)g0 * NN, g_tw256, ng, xs);
+			HIPCHK(hipGetLastError());
+		}
 		return 0;
 	}
 
@@ -855,7 +907,7 @@ int make_engine(int maxCells, qbf_engine** out)
 	e->gridN = N;
 	e->precision = std::is_same<R, float>::value ? QBF_FP32 : QBF_FP64;
 	e->maxCells = maxCells;
-	if (std::is_same<R, float>::value) {
+	if (std::is_same<R, float>::value && N != 128) {
 		// the paired path deviates from the complex one by 0.15 (Nyquist content of smooth states) to
 		// 0.8 (grid-scale noise) times the measured ratio; below 2e-5 that stays under the FP32 path's
 		// own deviation from FP64 (2.1e-5 L1 at N = 64, J = 32, M = 16)
@@ -888,6 +940,8 @@ int qbf_engine_create(int N, int precision, int maxCells, qbf_engine** out)
 	else if (precision == QBF_FP32 && N == 16) r = make_engine<float, 16>(maxCells, &e);
 	else if (precision == QBF_FP32 && N == 32) r = make_engine<float, 32>(maxCells, &e);
 	else if (precision == QBF_FP32 && N == 64) r = make_engine<float, 64>(maxCells, &e);
+	else if (precision == QBF_FP64 && N == 128) r = make_engine<double, 128>(maxCells, &e);
+	else if (precision == QBF_FP32 && N == 128) r = make_engine<float, 128>(maxCells, &e);
 	else return set_err(-1, "unsupported (N=%d, precision=%d)", N, precision);
 	if (r) return r;
 	hipStream_t s;
@@ -1123,6 +1177,7 @@ int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, d
 int qbf_set_real_pairs(qbf_engine* e, int mode, double tol)
 {
 	if (!e || mode < 0 || mode > 2 || !(tol >= 0)) return set_err(-1, "qbf_set_real_pairs: bad argument");
+	if (e->gridN == 128 && mode != 0) return set_err(-1, "qbf_set_real_pairs: no paired path at N = 128");
 	e->pairMode = mode;
 	e->pairTol = tol;
 	return 0;

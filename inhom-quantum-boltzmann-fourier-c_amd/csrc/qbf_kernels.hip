@@ -412,20 +412,21 @@ __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 // ---------------------------------------------------------------------------------------------
 // N x N transforms of the state (src/simulation.c:224-283): physical real -> Fourier / N^2
 // (full complex FFT of the real data), and Fourier -> unnormalised backward FFT -> real part.
-template <int N, int T>
+// XG: exchange through the per-workgroup slot xs + blockIdx.x * Xpose::SIZE (N = 128)
+template <int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_phys2fourier(const double* __restrict__ in, double2* __restrict__ out,
-	const double2* tw256, int ngrid)
+	const double2* tw256, int ngrid, double2* xs)
 {
 	using GE = Geo<N, T>;
 	constexpr int E = GE::E, G = GE::G;
-	__shared__ cx<double> lds[Xpose<double, GE>::SIZE];
+	__shared__ cx<double> lds[XG ? 1 : Xpose<double, GE>::SIZE];
 	__shared__ cx<double> tw[TwTab<N, T>::SIZE];
 	load_twiddles<double, N, T>(tw, tw256);
 	const int gi = blockIdx.x;
 	if (gi >= ngrid) return;
 	const int line = threadIdx.x / G, p = threadIdx.x % G;
 	const double* src = in + (size_t)gi * (N * N);
-	Xpose<double, GE> X{lds};
+	Xpose<double, GE> X{XG ? xs + (size_t)blockIdx.x * Xpose<double, GE>::SIZE : lds};
 	cx<double> v[E];
 #pragma unroll
 	for (int s = 0; s < E; s++) v[s] = make_double2(src[(p + G * s) * N + line], 0.0);
@@ -438,20 +439,20 @@ __global__ __launch_bounds__(T) void k_phys2fourier(const double* __restrict__ i
 	for (int u = 0; u < E; u++) dst[line * N + k0 + u] = make_double2(v[u].x * sc, v[u].y * sc);
 }
 
-template <int N, int T>
+template <int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_fourier2phys(const double2* __restrict__ in, double* __restrict__ out,
-	const double2* tw256, int ngrid)
+	const double2* tw256, int ngrid, double2* xs)
 {
 	using GE = Geo<N, T>;
 	constexpr int E = GE::E, G = GE::G;
-	__shared__ cx<double> lds[Xpose<double, GE>::SIZE];
+	__shared__ cx<double> lds[XG ? 1 : Xpose<double, GE>::SIZE];
 	__shared__ cx<double> tw[TwTab<N, T>::SIZE];
 	load_twiddles<double, N, T>(tw, tw256);
 	const int gi = blockIdx.x;
 	if (gi >= ngrid) return;
 	const int line = threadIdx.x / G, p = threadIdx.x % G;
 	const double2* src = in + (size_t)gi * (N * N);
-	Xpose<double, GE> X{lds};
+	Xpose<double, GE> X{XG ? xs + (size_t)blockIdx.x * Xpose<double, GE>::SIZE : lds};
 	cx<double> v[E];
 	const int k0 = E * brev<G>(p);
 #pragma unroll
