@@ -17,8 +17,10 @@
 namespace qbf {
 namespace big {
 constexpr int N = 128, P = 256, G = 16, E = 16;
-// threads per workgroup: FP64 lines need twice the registers, so half the threads (256 VGPRs a lane)
-template <typename R> constexpr int TB = sizeof(R) == 8 ? 512 : 1024;
+// threads per workgroup: 512 (8 waves, up to 256 VGPRs a lane: the line FFTs of the FP64 grids
+// and the instance sums of the I3 kernel fit without spills)
+template <typename R> constexpr int TB = 512;
+constexpr int I3_NI = 4;                    // scratch grids per I3 workgroup (instances of an output)
 constexpr int TW = 2 * P + G;               // twiddle table entries (load_twiddles_g)
 constexpr size_t PP = (size_t)P * P;
 static_assert(2 * N == P && G * E == P, "geometry");
@@ -268,9 +270,10 @@ __global__ __launch_bounds__(big::TB<R>) void k_i2acc_big(I2Args a)
 
 // ---------------------------------------------------------------------------------------------
 // acc[cell][out][chunk] (+)= sum_q sum_inst w F[f] .* K_q[Shat[g] .* F[h]],
-// K_q[X] = FFT2(psi2_q .* IFFT2(X))  (I3Args as k_i3acc, complex path).  Per application: rows
-// (IFFT along k of S .* H), columns (IFFT along l, times psi2_q, FFT along l), rows (FFT along
-// k, times w F, added to the accumulator slot).
+// K_q[X] = FFT2(psi2_q .* IFFT2(X))  (I3Args as k_i3acc, complex path).  All instances of the
+// output (at most 4) go through each pass together, in NI = 4 scratch grids per workgroup: rows
+// (IFFT along k of S .* H), columns (IFFT along l, times psi2_q read once, FFT along l), rows (FFT
+// along k, times w F, summed over the instances and added to the accumulator slot once per q).
 template <typename R>
 __global__ __launch_bounds__(big::TB<R>) void k_i3acc_big(I3Args a)
 {
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(big::TB<R>) void k_i3acc_big(I3Args a)
 	const int qa = (int)(((long long)chunk * a.nq) / a.nchunk);
 	const int qb = (int)(((long long)(chunk + 1) * a.nq) / a.nchunk);
 	const int lt = threadIdx.x / G, p = threadIdx.x % G;
-	cx<R>* xs = (cx<R>*)a.xs + (size_t)blockIdx.x * PP;
+	cx<R>* xs = (cx<R>*)a.xs + (size_t)blockIdx.x * (I3_NI * PP);
 	cx<R>* accp = (cx<R>*)a.acc + (((size_t)cell * a.nout + out) * a.nchunk + chunk) * PP;
 	const cx<R>* Sh = (const cx<R>*)a.Shat + (size_t)cell * a.nS * PP;
 	const int ni = a.ninst[out];
@@ -299,61 +302,78 @@ __global__ __launch_bounds__(big::TB<R>) void k_i3acc_big(I3Args a)
 			for (int u = 0; u < E; u++) row[u] = czero<R>();
 		}
 	__syncthreads();
+#pragma unroll 1
 	for (int qq = qa; qq < qb; qq++) {
 		const cx<R>* Fq = (const cx<R>*)a.F + ((size_t)cell * a.nq + qq) * a.nF * PP;
 		const R* ps = (const R*)a.psi2n + (size_t)(a.q0 + qq) * PP;
-		for (int in = 0; in < ni; in++) {
-			const cx<R>* Sg = Sh + (size_t)a.instS[out][in] * PP;
-			const cx<R>* Fh = Fq + (size_t)a.instH[out][in] * PP;
-			const cx<R>* Ff = Fq + (size_t)a.instF[out][in] * PP;
-			const R w = (R)a.instW[out][in];
 #pragma unroll 1
-			for (int sw = 0; sw < P / L; sw++) {
-				const int rb = (sw * L + lt) * P;
+		for (int sw = 0; sw < P / L; sw++) {
+			const int rb = (sw * L + lt) * P;
+#pragma unroll 1
+			for (int in = 0; in < ni; in++) {
+				const cx<R>* Sg = uni(Sh, (long long)a.instS[out][in] * PP) + rb + r0;
+				const cx<R>* Fh = uni(Fq, (long long)a.instH[out][in] * PP) + rb + r0;
 				cx<R> v[E];
 #pragma unroll
-				for (int u = 0; u < E; u++) v[u] = cmul<R>(Sg[rb + r0 + u], Fh[rb + r0 + u]);
+				for (int u = 0; u < E; u++) v[u] = cmul<R>(Sg[u], Fh[u]);
 				line_bwd<R, P, G, E>(v, p, tw);
+				cx<R>* x = uni(xs, (long long)in * PP) + rb + p;
 #pragma unroll
-				for (int s = 0; s < E; s++) xs[rb + p + G * s] = v[s];
+				for (int s = 0; s < E; s++) x[G * s] = v[s];
 			}
-			__syncthreads();
+		}
+		__syncthreads();
 #pragma unroll 1
-			for (int sw = 0; sw < P / L; sw++) {
-				const int c = sw * L + lt;
+		for (int sw = 0; sw < P / L; sw++) {
+			const int c = sw * L + lt;
+			R m[E];
+#pragma unroll
+			for (int s = 0; s < E; s++) m[s] = ps[(p + G * s) * P + c];
+#pragma unroll 1
+			for (int in = 0; in < ni; in++) {
+				cx<R>* x = uni(xs, (long long)in * PP) + r0 * P + c;
 				cx<R> v[E];
 #pragma unroll
-				for (int u = 0; u < E; u++) v[u] = xs[(r0 + u) * P + c];
+				for (int u = 0; u < E; u++) v[u] = x[u * P];
 				line_bwd<R, P, G, E>(v, p, tw);
 #pragma unroll
 				for (int s = 0; s < E; s++) {
-					const R m = ps[(p + G * s) * P + c];
-					v[s].x *= m;
-					v[s].y *= m;
+					v[s].x *= m[s];
+					v[s].y *= m[s];
 				}
 				line_fwd<R, P, G, E>(v, p, tw);
 #pragma unroll
-				for (int u = 0; u < E; u++) xs[(r0 + u) * P + c] = v[u];
+				for (int u = 0; u < E; u++) x[u * P] = v[u];
 			}
-			__syncthreads();
+		}
+		__syncthreads();
 #pragma unroll 1
-			for (int sw = 0; sw < P / L; sw++) {
-				const int rb = (sw * L + lt) * P;
+		for (int sw = 0; sw < P / L; sw++) {
+			const int rb = (sw * L + lt) * P;
+			cx<R> sum[E];
+#pragma unroll
+			for (int u = 0; u < E; u++) sum[u] = czero<R>();
+#pragma unroll 1
+			for (int in = 0; in < ni; in++) {
+				const cx<R>* x = uni((const cx<R>*)xs, (long long)in * PP) + rb + p;
 				cx<R> v[E];
 #pragma unroll
-				for (int s = 0; s < E; s++) v[s] = xs[rb + p + G * s];
+				for (int s = 0; s < E; s++) v[s] = x[G * s];
 				line_fwd<R, P, G, E>(v, p, tw);
+				const cx<R>* Ff = uni(Fq, (long long)a.instF[out][in] * PP) + rb + r0;
+				const R w = (R)a.instW[out][in];
 #pragma unroll
 				for (int u = 0; u < E; u++) {
-					const cx<R> y = cmul<R>(Ff[rb + r0 + u], v[u]);
-					cx<R> t = accp[rb + r0 + u];
-					t.x += w * y.x;
-					t.y += w * y.y;
-					accp[rb + r0 + u] = t;
+					const cx<R> y = cmul<R>(Ff[u], v[u]);
+					sum[u].x += w * y.x;
+					sum[u].y += w * y.y;
 				}
 			}
-			__syncthreads();
+			cx<R>* row = accp + rb + r0;
+#pragma unroll
+			for (int u = 0; u < E; u++) row[u] = cadd<R>(row[u], sum[u]);
 		}
+		__syncthreads();
 	}
 }
 

@@ -60,10 +60,10 @@ template <> struct Cfg<float, 32> { static constexpr int T = 256; static constex
 #define QBF_F32_N64_THREADS 1024   // 512 also fits in registers (G = 4, E = 32)
 #endif
 template <> struct Cfg<float, 64> { static constexpr int T = QBF_F32_N64_THREADS; static constexpr bool XG = false; };
-// N = 128: the padded grids stay in global memory (qbf_big.hip: one workgroup of 1024 (FP32) or 512
-// (FP64) threads per grid, line sweeps; T must match big::TB, XG sizes the scratch per grid)
+// N = 128: the padded grids stay in global memory (qbf_big.hip: one workgroup of 512 threads per
+// grid, line sweeps; T must match big::TB, XG sizes the scratch per grid)
 template <> struct Cfg<double, 128> { static constexpr int T = 512; static constexpr bool XG = true; };
-template <> struct Cfg<float, 128> { static constexpr int T = 1024; static constexpr bool XG = true; };
+template <> struct Cfg<float, 128> { static constexpr int T = 512; static constexpr bool XG = true; };
 template <int N> struct CfgNN;
 template <> struct CfgNN<16> { static constexpr int T = 64; static constexpr bool XG = false; };
 template <> struct CfgNN<32> { static constexpr int T = 128; static constexpr bool XG = false; };
@@ -324,11 +324,11 @@ struct EngineT final : public qbf_engine {
 
 	// --------------------------------------------------------------- launches
 	// exchange scratch for nblocks workgroups (only for configurations that exchange through global memory)
-	int xs_for(int nblocks, void** xs)
+	int xs_for(int nblocks, void** xs, int gridsPerBlock = 1)
 	{
 		*xs = nullptr;
 		if (!XG) return 0;
-		const size_t per = BIG ? (size_t)PP : (size_t)Xpose<R, GE>::SIZE;
+		const size_t per = BIG ? (size_t)PP * gridsPerBlock : (size_t)Xpose<R, GE>::SIZE;
 		int r = xscr.ensure((size_t)nblocks * per * sizeof(C));
 		if (r) return r;
 		*xs = xscr.p;
@@ -456,7 +456,7 @@ struct EngineT final : public qbf_engine {
 				for (int i = 0; i < (pair ? 2 * ninst[o] : ninst[o]); i++) a.instW[o][i] = iW[o][i];
 			}
 			a.nchunk = nchunk; a.accumulate = q0 > 0; a.acc = acc3.p; a.tw256 = g_tw256;
-			r = xs_for(ncell * nout * nchunk, &a.xs);
+			r = xs_for(ncell * nout * nchunk, &a.xs, BIG ? big::I3_NI : 1);
 			if (r) return r;
 			hipEvent_t eb = nullptr, ee = nullptr;
 			if (prof && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
