@@ -110,6 +110,9 @@ def main():
         print("bench.py is a single-process, single-GPU benchmark")
         sys.exit(2)
     paths = set(args.paths.split(","))
+    if args.N != 64 and "surrogate" in paths:
+        print("the CNN surrogate is built for N = 64 only; running the exact paths", file=sys.stderr)
+        paths.discard("surrogate")
 
     N, Nx, J, M = args.N, args.Nx, args.J, args.M
     cpu = None
@@ -144,24 +147,35 @@ def main():
         if (int(z["N"]), int(z["Nx"]), int(z["J"]), int(z["M"]), int(z["seed"])) == (N, Nx, J, M, args.seed):
             ref = z
 
+    free_at_create, footprint = {}, {}
+
     def make(prec):
+        free_at_create[prec] = torch.cuda.mem_get_info()[0]
         e = qbf.Engine(N, prec, max_cells=Nx)
         e.set_quadrature(*tabs)
         return e
+
+    def note_footprint(prec):
+        """device memory taken since the engine was created (its tables, intermediate grids and scratch)"""
+        torch.cuda.synchronize()
+        footprint[prec] = (free_at_create[prec] - torch.cuda.mem_get_info()[0]) / 2 ** 30
 
     def run_steps(step, W, k, keep_at=()):
         """k steps from W with two alternating buffers; returns the final state and copies of the
         states after the steps listed in keep_at."""
         bufs = [W.clone(), torch.empty_like(W)]
         cur, kept = 0, {}
+        t_last = time.perf_counter()
         for i in range(1, k + 1):
             step(bufs[cur], bufs[1 - cur])
             cur = 1 - cur
             if i in keep_at:
                 kept[i] = bufs[cur].clone()
-            if k >= 50 and i % 25 == 0:          # long runs report their progress (stderr)
+            # long runs report their progress (stderr): every 25 steps, or after steps of > 10 s
+            if (k >= 50 and i % 25 == 0) or time.perf_counter() - t_last > 10:
                 torch.cuda.synchronize()
                 print(f"  {i}/{k} steps", file=sys.stderr, flush=True)
+                t_last = time.perf_counter()
         return bufs[cur], kept
 
     def moment_errors(W, step_no):
@@ -191,7 +205,7 @@ def main():
     if "mixed" in paths:
         step32 = lambda a, b: e32.inhom_step_periodic(a, wl.L_BOX, h, dt, out=b)  # noqa: E731
         W32_1, _ = run_steps(step32, W0, 1)
-        torch.cuda.synchronize()
+        note_footprint(qbf.FP32)
         e32.profile(True)
         t0 = time.perf_counter()
         W32_n, _ = run_steps(step32, W32_1, args.mixed_steps)
@@ -209,7 +223,7 @@ def main():
         e64 = make(qbf.FP64)
         step64 = lambda a, b: e64.inhom_step_periodic(a, wl.L_BOX, h, dt, out=b)  # noqa: E731
         W64_1, _ = run_steps(step64, W0, 1)
-        torch.cuda.synchronize()
+        note_footprint(qbf.FP64)
         t0 = time.perf_counter()
         run_steps(step64, W64_1, args.fp64_steps)
         torch.cuda.synchronize()
@@ -292,8 +306,12 @@ def main():
                   "note": "paired I3 path: one complex K_q application carries two real ones; the roofline "
                           "counts executed applications"}
     achieved = kapps * flop_per_k / (kms / 1e3) if kms > 0 else 0.0
+    headline = (N, Nx) == (64, 512)
+    i3_kernel = ("k_i3acc<float,64,1024,false,%s> (I3 K-operator: IFFT2 -> psi2 -> FFT2)" % ("true" if used else "false")
+                 if N == 64 else f"k_i3acc_big<float> (I3 K-operator on {P} x {P} grids in global memory)" if N == 128
+                 else f"k_i3acc<float,{N},...> (I3 K-operator)")
     line = {
-        "metric": METRIC,
+        "metric": METRIC if headline else f"collision evals/s at N={N},Nx={Nx}: FP32 exact vs FP64 plain",
         "value": round(F["value"], 3),
         "unit": "collision evals/s",
         "n_gpus": 1,
@@ -313,13 +331,16 @@ def main():
                                   "FP32 convolution kernels, FP64 state, midpoint update and conservation correction"),
                    "path": fast, "N": N, "Nx": Nx, "J": J, "M": M, "evals_per_step": evals_per_step,
                    "i3_pairs": pairs_info},
-        "roofline": {"bound": "mfma", "kernel": "k_i3acc<float,64,1024,false,%s> (I3 K-operator: IFFT2 -> psi2 -> FFT2)" % ("true" if used else "false"),
+        "roofline": {"bound": "mfma", "kernel": i3_kernel,
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": I3_TRAFFIC_BYTES[bool(used)],
+                     "frac": round(achieved / PEAK["fp32"], 4),
+                     "traffic": I3_TRAFFIC_BYTES[bool(used)] if N == 64 else None,
                      "launches": nl, "avg_launch_ms": round(kms / max(1, nl), 4),
                      "kernel_share_of_timed": round(kms / 1e3 / twall, 4)},
         "cpu_baseline": cpu,
     }
+    if footprint:
+        line["hbm_footprint_gib"] = {("fp32" if k == qbf.FP32 else "fp64"): round(v, 2) for k, v in footprint.items()}
     if "fp64" in out:
         line["fp64_plain"] = {"value": round(out["fp64"]["value"], 3), "unit": "collision evals/s",
                               "ms_per_step": round(out["fp64"]["ms_per_step"], 3), "steps": out["fp64"]["steps"]}

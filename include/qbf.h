@@ -44,6 +44,8 @@ typedef struct {
 
 const char *qbf_last_error(void);
 
+/* N = 16, 32, 64 (padded grids held in registers/LDS by one workgroup) or 128 (padded 256 x 256 grids
+ * in global memory; no paired path).  The reference fixes N at compile time (include/grid.h:34). */
 int  qbf_engine_create(int N, int precision, int maxCells, qbf_engine **out);
 void qbf_engine_destroy(qbf_engine *e);
 int  qbf_engine_N(const qbf_engine *e);

@@ -350,20 +350,6 @@ void InhomStepPeriodic(const wignerV_t *restrict_ W, const quadI1_t *restrict_ q
 		report("InhomStepPeriodic");
 }
 
-/* Collision at cells [c0, c1) of a host physical state, on the GPU (src/simulation.c:214-284). */
-static int collide_cells(wignerV_t *Wt, unsigned c0, unsigned c1, double dt, const double *Bext)
-{
-	if (c1 <= c0) return 0;
-	const int nc = (int)(c1 - c0);
-	const size_t sz = nc * sizeof(wignerV_t);
-	double *dP = dbuf(0, sz), *dF = dbuf(3, 2 * sz), *dF2 = dbuf(4, 2 * sz), *dB = dbuf(2, 3 * nc * sizeof(double));
-	if (!dP || !dF || !dF2 || !dB) return -1;
-	if (qbf_h2d(g_eng, dP, Wt + c0, sz) || qbf_h2d(g_eng, dB, Bext + 3 * c0, 3 * nc * sizeof(double)) ||
-		qbf_phys_to_fourier(g_eng, dP, 4 * nc, dF) || qbf_collision_step(g_eng, dF, nc, dt, dB, dF2) ||
-		qbf_fourier_to_phys(g_eng, dF2, 4 * nc, dP) || qbf_d2h(g_eng, Wt + c0, dP, sz)) return -1;
-	return 0;
-}
-
 void InhomStepDirichlet(const wignerV_t *restrict_ W, const quadI1_t *restrict_ quadI1, const quadI2_t *restrict_ quadI2,
 	const quadI3_t *restrict_ quadI3, const quadI1_t *restrict_ quadI4,
 	const double h, const double dt, const double *restrict_ Bext, inhomStepInterm_t *restrict_ interm, wignerV_t *restrict_ Wnext)

@@ -83,6 +83,23 @@ def test_host_quadrature_matches_oracle():
         assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
 
 
+def test_host_quadrature_matches_oracle_n128():
+    import oracle as O
+    import qbf
+    a = qbf.quadrature(128, 4, 6.0, 3.75, 2)
+    q = O.Quad(128, 4, 6.0, 3.75, 2)
+    for x, y in zip(a, (q.I1, q.I2, q.I3, q.I4)):
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+
+
+@pytest.mark.parametrize("ngrid", [64, 128])
+def test_dropin_layer_compiles_for_other_grid_sizes(ngrid, tmp_path):
+    """The drop-in layer follows the reference's compile-time N_GRID (include/grid.h)."""
+    src = os.path.join(ROOT, "inhom-quantum-boltzmann-fourier-c_amd", "csrc", "qbf_compat.c")
+    subprocess.run(["gcc", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-Werror", "-Wno-unused-parameter", "-Wno-comment",
+                    f"-DN_GRID={ngrid}", "-I" + INC, "-c", src, "-o", str(tmp_path / "compat.o")], check=True)
+
+
 def test_product_fails_loudly_without_gpu():
     import torch
     import qbf
