@@ -1,4 +1,4 @@
-"""The two FP64 plain configurations at their real sizes (velocity grid 32 x 32 on [-6, 6)^2,
+"""The configurations at their real sizes.  The two FP64 plain ones (velocity grid 32 x 32 on [-6, 6)^2,
 Gauss-Legendre order 16 for I3):
 
 * homogeneous, two seeded Gaussians, midpoint collision step dt = 0.01, 200 steps
@@ -10,7 +10,9 @@ Gauss-Legendre order 16 for I3):
   src/simulation.c:1288-1301): the first Strang step against the oracle on all 128 cells (1e-12;
   quadrature J = 8, M = 4, which keeps the oracle at seconds), then 100 steps with J = 32, M = 16 on
   the GPU: the device-resident loop equals repeated single steps, total mass drifts by less than
-  1e-8, eigenvalues stay in [0, 1] to 1e-5.
+  1e-8, eigenvalues stay in [0, 1] to 1e-5;
+
+and the scale configuration's shape (N = 128, Nx = 2048), see its test.
 """
 import numpy as np
 import pytest
@@ -99,3 +101,47 @@ def test_periodic_config_128_cells_100_steps():
     print(f"mass drift after 99 steps {abs(m1 - m0) / m0:.3e}, eigenvalues {lo:.3e} .. {hi:.6f}")
     assert abs(m1 - m0) < 1e-8 * m0                                   # measured 3.8e-10
     assert lo > -1e-5 and hi < 1 + 1e-5                               # measured -6.6e-7 .. 0.87
+
+
+def test_scale_config_n128_2048_cells():
+    """The scale configuration's shape: N = 128 (padded 256 x 256), Nx = 2048 cells, generator modes
+    k <= 8, FP32 convolutions with the FP64 conservation correction, 4 device-resident steps.  The
+    quadrature is J = 4, M = 2 here (seconds); the full J = 32, M = 16 step is timed by bench.py
+    (--N 128 --Nx 2048).  Checks: the device-resident loop equals repeated single steps, total mass
+    and energy are conserved (transport in flux form, collision corrected), eigenvalues stay in
+    [0, 1], and the first FP32 step agrees with the FP64 step (both conservation-corrected) on all 2048
+    cells."""
+    import torch
+    import qbf
+    from qbf import workloads as wl
+    N128, Nx = 128, 2048
+    L, h, dt = wl.L_BOX, 1.0 / Nx, wl.cfl_dt(N128, Nx)
+    tabs = qbf.quadrature(N128, 4, L, wl.R_BALL, 2)
+    W0 = wl.periodic(N128, Nx, seed=5, kmax=8)
+    Wd = torch.from_numpy(W0).cuda()
+    print("N=128, Nx=2048: FP32 engine, 4 device-resident steps", flush=True)
+    e = qbf.Engine(N128, qbf.FP32, max_cells=Nx)
+    e.set_quadrature(*tabs)
+    e.set_moment_correction(qbf.CONS_ALL)
+    frames = e.simulate_periodic(Wd, L, h, dt, 4)
+    assert frames.shape[0] == 4
+    w1 = e.inhom_step_periodic(Wd, L, h, dt).cpu().numpy()
+    assert np.array_equal(w1, frames[1])
+    e.close()
+    m = np.stack([wl.moments(f).sum(axis=0) for f in frames])      # [step][mass, spin3, momentum2, energy]
+    dm = np.abs(m[:, 0] - m[0, 0]).max() / m[0, 0]
+    de = np.abs(m[:, 6] - m[0, 6]).max() / m[0, 6]
+    lo, hi = _eigen_range(frames[-1])
+    print(f"mass drift {dm:.2e}, energy drift {de:.2e}, eigenvalues {lo:.2e} .. {hi:.6f}", flush=True)
+    assert dm < 1e-12 and de < 1e-12
+    assert lo > -1e-5 and hi < 1 + 1e-5
+    del frames
+    print("N=128, Nx=2048: FP64 step (same conservation correction)", flush=True)
+    e = qbf.Engine(N128, qbf.FP64, max_cells=Nx)
+    e.set_quadrature(*tabs)
+    e.set_moment_correction(qbf.CONS_ALL)
+    w64 = e.inhom_step_periodic(Wd, L, h, dt).cpu().numpy()
+    e.close()
+    err = l1rel(w1, w64)
+    print(f"FP32 vs FP64 after one step: {err:.2e} (increment {np.abs(w1 - w64).sum() / np.abs(w64 - W0).sum():.2e})")
+    assert err < 1e-7
