@@ -177,9 +177,9 @@ void qbf_fourier_I3(int N, unsigned J, double L, double R, unsigned M, double *p
 void qbf_fourier_I4(int N, unsigned J, double L, double R, double *psiR1, double *psiR2);
 
 /* ---- CNN surrogate of the collision operator (no counterpart in the reference) ----
- * Six periodic 3x3 convolutions 4 -> 64 -> 64 -> 64 -> 64 -> 64 -> 4 on 64 x 64 grids, GELU
- * between them, bf16 weights and activations, fp32 accumulation on the matrix cores.
- * Inputs / outputs: planar fp32 [B][4][64][64] device arrays.  Parameters (host, fp32):
+ * Six periodic 3x3 convolutions 4 -> 64 -> 64 -> 64 -> 64 -> 64 -> 4 on N x N grids (N = 64 or
+ * 128), GELU between them, bf16 weights and activations, fp32 accumulation on the matrix cores.
+ * Inputs / outputs: planar fp32 [B][4][N][N] device arrays.  Parameters (host, fp32):
  * per layer weights [9][cout_pad][cin_pad] then bias [cout_pad]; cin_pad = 32 for layer 0
  * (4 used), cout_pad = 32 for layer 5 (4 used), 64 otherwise; unused entries are zero. */
 typedef struct qbf_cnn qbf_cnn;
@@ -196,12 +196,13 @@ int    qbf_cnn_set_params(qbf_cnn *c, const float *host);
 int    qbf_cnn_get_params(qbf_cnn *c, float *host, int grads);
 /* out = oscale * net(iscale * in) */
 int    qbf_cnn_forward(qbf_cnn *c, const float *dIn, int B, float iscale, float *dOut, float oscale);
-/* forward + backward of loss = mean((out - target)^2); gradients stay on the device */
+/* forward + backward of loss = mean((out - target)^2); gradients stay on the device;
+ * B <= min(maxBatch, 256) (the training buffers are sized for that) */
 int    qbf_cnn_gradients(qbf_cnn *c, const float *dIn, const float *dTarget, int B, float iscale, float oscale,
 	float *dOut, float *hostLoss);
 int    qbf_cnn_adam(qbf_cnn *c, float lr, float b1, float b2, float eps);
 /* Midpoint collision step with C(W) ~ oscale * net(iscale * W) on physical states
- * [ncell][4][64][64] (device doubles), W_half = W + dt/2 C(W), out = W + dt C(W_half), then the
+ * [ncell][4][N][N] (device doubles, the engine's N = the network's), W_half = W + dt/2 C(W), out = W + dt C(W_half), then the
  * FP64 moment correction of out - W (mask QBF_CONS_*, 0 = none).  Launches on e's stream. */
 int    qbf_surrogate_collision_step(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double dt,
 	float iscale, float oscale, double L, int mask, double *dOut);

@@ -6,9 +6,10 @@
 // stored NHWC in bf16 (channels padded: 4 -> 32 at the input, 4 -> 32 at the output), weights
 // as bf16 copies of fp32 master weights, and every convolution accumulates in fp32.
 //
-// Forward and backward-data convolutions are implicit GEMMs on the matrix cores
-// (v_mfma_f32_16x16x32_bf16): a workgroup of 4 waves owns one image row (S = 64 pixels),
-// each wave 16 pixels times all output channels; K = 9 taps x input channels.
+// The image side S is 64 or 128 (velocity grids N = 64, 128; a template parameter of the
+// kernels).  Forward and backward-data convolutions are implicit GEMMs on the matrix cores
+// (v_mfma_f32_16x16x32_bf16): each wave owns a 64-pixel segment of an image row times all output
+// channels; K = 9 taps x input channels.
 // MFMA operand layout (16 x 16 x 32): lane l holds A[l%16][8*(l/16) .. +7],
 // B[8*(l/16) .. +7][l%16], and the result D[4*(l/16) + r][l%16], r < 4.
 // The weight gradient (a reduction over all pixels) runs on the VALU in fp32.
@@ -31,7 +32,7 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __bf16 bf16;
 
-constexpr int S = 64;           // image side (velocity grid N)
+constexpr int SEGP = 64;        // pixels of an image row segment (one wave of the convolutions)
 constexpr int NL = 6;           // layers
 constexpr int CH = 64;          // hidden channels
 constexpr int CIO = 32;         // padded input / output channel count
@@ -65,6 +66,7 @@ __device__ __forceinline__ float gelu_grad(float z)
 }
 
 // a[b] = max |x[b][0][.]|  (one workgroup of 256 per cell)
+template <int S>
 __global__ __launch_bounds__(256) void k_cell_amp(const float* __restrict__ x, float* __restrict__ amp)
 {
 	__shared__ float sh[4];
@@ -79,6 +81,7 @@ __global__ __launch_bounds__(256) void k_cell_amp(const float* __restrict__ x, f
 
 // planar fp32 [B][4][S][S] -> NHWC bf16 [B][S][S][CIO]: channels 0..3 = scale * x (or
 // scale * x / a[b] with amp), channel 4 = scale * a[b] with amp, the rest zero
+template <int S>
 __global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, int B, float scale,
 	const float* __restrict__ amp)
 {
@@ -103,23 +106,26 @@ __global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, i
 // MODE 1: out = bias + value, store fp32 planar [B][4][S][S] (first 4 channels), times
 //         oscale (and a[b]^2 when amp is given)
 // MODE 2: backward data: g = value * gelu'(zprev) -> bf16 NHWC (gradient of the previous z)
-// A workgroup of CROWS waves computes CROWS image rows; the layer's weights are staged once in
-// LDS (row pitch KC + 4 elements: the 16 lanes of a 16-byte read cover all 64 banks but one
-// pair).  Each wave holds a 64-pixel x NC accumulator (4 x NC/16 MFMA tiles): per tap and
-// 32-channel block it loads 4 A fragments from global memory and NC/16 B fragments from LDS
-// for 4 * NC/16 MFMAs.
+// A workgroup of CROWS waves computes CROWS consecutive row segments (64 pixels each: S / 64
+// per image row); the layer's weights are staged once in LDS (row pitch KC + 4 elements: the 16
+// lanes of a 16-byte read cover all 64 banks but one pair).  Each wave holds a 64-pixel x NC
+// accumulator (4 x NC/16 MFMA tiles): per tap and 32-channel block it loads 4 A fragments from
+// global memory and NC/16 B fragments from LDS for 4 * NC/16 MFMAs.
 constexpr int CROWS = 8;
-template <int KC, int NC, int DIR, int MODE>
+template <int S, int KC, int NC, int DIR, int MODE>
 __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ Wt,
 	const float* __restrict__ bias, float* __restrict__ zout, bf16* __restrict__ aout, float* __restrict__ yout,
 	const float* __restrict__ zprev, float oscale, int dil, const float* __restrict__ amp)
 {
 	constexpr int NT = NC / 16;       // 16-wide output-channel tiles
-	constexpr int MT = S / 16;        // 16-pixel tiles of a row
+	constexpr int MT = SEGP / 16;     // 16-pixel tiles of a segment
+	constexpr int SEG = S / SEGP;     // segments per image row
 	constexpr int KP = KC + 4;        // LDS pitch of a weight row (elements)
 	__shared__ __attribute__((aligned(16))) bf16 wl[9 * NC * KP];
-	const int b = blockIdx.x / (S / CROWS);
-	const int y = (blockIdx.x % (S / CROWS)) * CROWS + (threadIdx.x >> 6);
+	const int tile = blockIdx.x * CROWS + (threadIdx.x >> 6);
+	const int b = tile / (S * SEG);
+	const int y = (tile / SEG) % S;
+	const int x0 = (tile % SEG) * SEGP;
 	const int lane = threadIdx.x & 63;
 	const int i16 = lane & 15, kq = lane >> 4;
 	for (int v = threadIdx.x; v < 9 * NC * (KC / 8); v += 64 * CROWS) {
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 		const int sh = DIR * dil * (tap % 3 - 1) + 2 * S;
 		const bf16* rowp = in + ((size_t)b * S + ys) * S * KC + kb * 32 + kq * 8;
 #pragma unroll
-		for (int m = 0; m < MT; m++) a[m] = *(const bf16x8*)(rowp + (size_t)((m * 16 + i16 + sh) & (S - 1)) * KC);
+		for (int m = 0; m < MT; m++) a[m] = *(const bf16x8*)(rowp + (size_t)((x0 + m * 16 + i16 + sh) & (S - 1)) * KC);
 	};
 	bf16x8 a[MT];
 	load_a(0, a);
@@ -174,19 +180,19 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 	float osc = oscale;
 	if (MODE == 1 && amp) osc *= amp[b] * amp[b];
 	if constexpr (MODE == 1) {
-		// the CREAL output channels of the row go through LDS (the weights are no longer needed)
-		// so that the NCHW float rows are written whole: [c][S] per wave, 16 bytes per lane
+		// the CREAL output channels of the segment go through LDS (the weights are no longer
+		// needed) so that the NCHW float rows are written whole: [c][64] per wave, 16 bytes a lane
 		__syncthreads();
-		float* ys = (float*)wl + (threadIdx.x >> 6) * (CREAL * S);
+		float* ys = (float*)wl + (threadIdx.x >> 6) * (CREAL * SEGP);
 		if (i16 < CREAL)
 #pragma unroll
 			for (int m = 0; m < MT; m++)
 #pragma unroll
-				for (int r = 0; r < 4; r++) ys[i16 * S + m * 16 + kq * 4 + r] = osc * (acc[m][0][r] + bias[i16]);
+				for (int r = 0; r < 4; r++) ys[i16 * SEGP + m * 16 + kq * 4 + r] = osc * (acc[m][0][r] + bias[i16]);
 		__syncthreads();
-		static_assert(CREAL * S == 4 * 64, "one float4 per lane");
-		const int c = lane / (S / 4), x4 = (lane % (S / 4)) * 4;
-		*(float4*)(yout + ((size_t)b * CREAL + c) * (S * S) + (size_t)y * S + x4) = *(const float4*)(ys + c * S + x4);
+		static_assert(CREAL * SEGP == 4 * 64, "one float4 per lane");
+		const int c = lane / (SEGP / 4), x4 = (lane % (SEGP / 4)) * 4;
+		*(float4*)(yout + ((size_t)b * CREAL + c) * (S * S) + (size_t)y * S + x0 + x4) = *(const float4*)(ys + c * SEGP + x4);
 		return;
 	}
 #pragma unroll
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 			const int n = t * 16 + i16;
 #pragma unroll
 			for (int r = 0; r < 4; r++) {
-				const int xo = m * 16 + kq * 4 + r;
+				const int xo = x0 + m * 16 + kq * 4 + r;
 				const size_t pix = ((size_t)b * S + y) * S + xo;
 				const float v = acc[m][t][r];
 				if (MODE == 0) {
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 // One workgroup per (image, block of ROWS rows); thread (co, 16-channel ci block) keeps
 // 9 x 16 accumulators; the three rows of x that a row of dz meets and that row of dz are
 // staged in LDS (periodic in x).
-template <int CI, int CO>
+template <int S, int CI, int CO>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dz,
 	float* __restrict__ dW, float* __restrict__ db, int rows, int dil)
 {
@@ -274,6 +280,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, 
 
 // loss = mean over elements of d^2, gradient with respect to the network output -> bf16 NHWC
 // (CIO channels).  d = pred - target, or (pred - target) / (oscale a[b]^2) with amp.
+template <int S>
 __global__ void k_loss_grad(const float* __restrict__ pred, const float* __restrict__ target, bf16* __restrict__ g,
 	float* __restrict__ loss, int B, float inv_n, float oscale, const float* __restrict__ amp)
 {
@@ -363,10 +370,10 @@ __global__ void k_update_f64(const double* __restrict__ W, const float* __restri
 	if (i < n) out[i] = W[i] + a * (double)Cv[i];
 }
 
-// gather cells idx[j] of W [*][4][S][S] into out [nsel][4][S][S]
-__global__ void k_gather_cells(const double* __restrict__ W, const int* __restrict__ idx, double* __restrict__ out, int nsel)
+// gather cells idx[j] of W [*][CELL] into out [nsel][CELL] (CELL = 4 S S values)
+__global__ void k_gather_cells(const double* __restrict__ W, const int* __restrict__ idx, double* __restrict__ out, int nsel,
+	int CELL)
 {
-	constexpr int CELL = CREAL * S * S;
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= (long long)nsel * CELL) return;
 	const int j = (int)(i / CELL), e = (int)(i % CELL);
@@ -426,7 +433,12 @@ static int cnn_err(int c, const char* m)
 	} while (0)
 
 struct qbf_cnn {
+	int S = 64;                      // image side (the velocity grid N: 64 or 128)
 	int maxB = 0;
+	// training (qbf_cnn_gradients) works on batches of at most min(maxB, TRAIN_MAX) images: the
+	// pre-activations and gradients are held for those only (inference batches may be larger)
+	static constexpr int TRAIN_MAX = 256;
+	int maxTrain = 0;
 	hipStream_t own = nullptr;       // created by qbf_cnn_create, destroyed by qbf_cnn_destroy
 	hipStream_t stream = nullptr;    // where work is launched (own, or the caller's)
 	// per layer: fp32 master weights / grads / Adam moments, bf16 copies
@@ -480,18 +492,22 @@ const char* qbf_cnn_last_error(void) { return g_cnn_err.c_str(); }
 
 int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 {
-	if (!out || N != S || maxBatch < 1) return cnn_err(-1, "qbf_cnn_create: N must be 64 and maxBatch >= 1");
+	if (!out || (N != 64 && N != 128) || maxBatch < 1)
+		return cnn_err(-1, "qbf_cnn_create: N must be 64 or 128 and maxBatch >= 1");
 	qbf_cnn* c = new qbf_cnn();
+	const int S = N;
+	c->S = N;
 	c->maxB = maxBatch;
-	const size_t px = (size_t)maxBatch * S * S;
+	c->maxTrain = maxBatch < qbf_cnn::TRAIN_MAX ? maxBatch : qbf_cnn::TRAIN_MAX;
+	const size_t px = (size_t)maxBatch * S * S, pxt = (size_t)c->maxTrain * S * S;
 	for (int l = 0; l < NL; l++) {
 		const size_t nw = wsize(l), nb = cout_of(l);
 		c->w[l] = c->alloc_f(nw); c->gw[l] = c->alloc_f(nw); c->mw[l] = c->alloc_f(nw); c->vw[l] = c->alloc_f(nw);
 		c->bias[l] = c->alloc_f(nb); c->gb[l] = c->alloc_f(nb); c->mb[l] = c->alloc_f(nb); c->vb[l] = c->alloc_f(nb);
 		c->wb[l] = c->alloc_b(nw); c->wbt[l] = c->alloc_b(nw);
 		c->x[l] = c->alloc_b(px * cin_of(l));
-		c->g[l] = c->alloc_b(px * cout_of(l));
-		if (l < NL - 1) c->z[l] = c->alloc_f(px * cout_of(l));
+		c->g[l] = c->alloc_b(pxt * cout_of(l));
+		if (l < NL - 1) c->z[l] = c->alloc_f(pxt * cout_of(l));
 		if (!c->w[l] || !c->gw[l] || !c->mw[l] || !c->vw[l] || !c->bias[l] || !c->gb[l] || !c->mb[l] || !c->vb[l] ||
 			!c->wb[l] || !c->wbt[l] || !c->x[l] || !c->g[l] || (l < NL - 1 && !c->z[l])) {
 			delete c;
@@ -609,26 +625,41 @@ int qbf_cnn_get_params(qbf_cnn* c, float* host, int grads)
 	return 0;
 }
 
-static int run_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale, bool train)
+} // extern "C"
+
+// convolution grid: CROWS row segments of 64 pixels per workgroup
+template <int S>
+static dim3 conv_grid(int B) { return dim3((unsigned)(B * S * (S / SEGP) / CROWS)); }
+
+template <int S>
+static int run_forward_t(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale, bool train)
 {
 	const long long npx = (long long)B * S * S;
 	const float* amp = c->norm ? c->amp : nullptr;
-	if (c->norm) hipLaunchKernelGGL(k_cell_amp, dim3(B), dim3(256), 0, c->stream, dIn, c->amp);
-	hipLaunchKernelGGL(k_to_nhwc, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B, iscale,
-		amp);
-	const dim3 grid(B * (S / CROWS)), blk(64 * CROWS);
-	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
+	if (c->norm) hipLaunchKernelGGL(k_cell_amp<S>, dim3(B), dim3(256), 0, c->stream, dIn, c->amp);
+	hipLaunchKernelGGL(k_to_nhwc<S>, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B,
+		iscale, amp);
+	const dim3 grid = conv_grid<S>(B), blk(64 * CROWS);
+	hipLaunchKernelGGL((k_conv_mfma<S, CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
 		train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f, c->dil[0], nullptr);
 	for (int l = 1; l < NL - 1; l++)
-		hipLaunchKernelGGL((k_conv_mfma<CH, CH, 1, 0>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
+		hipLaunchKernelGGL((k_conv_mfma<S, CH, CH, 1, 0>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
 			train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f, c->dil[l], nullptr);
-	hipLaunchKernelGGL((k_conv_mfma<CH, CIO, 1, 1>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
+	hipLaunchKernelGGL((k_conv_mfma<S, CH, CIO, 1, 1>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
 		c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale, c->dil[NL - 1], amp);
 	HCHK(hipGetLastError());
 	return 0;
 }
 
-// out = oscale * net(iscale * in), planar fp32 [B][4][64][64] device arrays
+extern "C" {
+
+static int run_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale, bool train)
+{
+	return c->S == 128 ? run_forward_t<128>(c, dIn, B, iscale, dOut, oscale, train)
+	                   : run_forward_t<64>(c, dIn, B, iscale, dOut, oscale, train);
+}
+
+// out = oscale * net(iscale * in), planar fp32 [B][4][N][N] device arrays
 int qbf_cnn_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dOut, float oscale)
 {
 	if (!c || !dIn || !dOut || B < 1 || B > c->maxB) return cnn_err(-1, "qbf_cnn_forward: bad argument");
@@ -636,11 +667,13 @@ int qbf_cnn_forward(qbf_cnn* c, const float* dIn, int B, float iscale, float* dO
 }
 
 // gradients of loss = mean (out - target)^2 into the gradient buffers (no update); loss -> *hostLoss
-int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B, float iscale, float oscale,
+} // extern "C"
+
+template <int S>
+static int gradients_t(qbf_cnn* c, const float* dIn, const float* dTarget, int B, float iscale, float oscale,
 	float* dOut, float* hostLoss)
 {
-	if (!c || !dIn || !dTarget || !dOut || B < 1 || B > c->maxB) return cnn_err(-1, "qbf_cnn_gradients: bad argument");
-	int r = run_forward(c, dIn, B, iscale, dOut, oscale, true);
+	int r = run_forward_t<S>(c, dIn, B, iscale, dOut, oscale, true);
 	if (r) return r;
 	for (int l = 0; l < NL; l++) {
 		HCHK(hipMemsetAsync(c->gw[l], 0, wsize(l) * sizeof(float), c->stream));
@@ -649,23 +682,23 @@ int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B,
 	HCHK(hipMemsetAsync(c->loss, 0, sizeof(float), c->stream));
 	const long long npx = (long long)B * S * S;
 	const float inv_n = 1.0f / (float)(npx * CREAL);
-	hipLaunchKernelGGL(k_loss_grad, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dOut, dTarget,
+	hipLaunchKernelGGL(k_loss_grad<S>, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dOut, dTarget,
 		c->g[NL - 1], c->loss, B, inv_n, oscale, c->norm ? c->amp : nullptr);
-	const dim3 grid(B * (S / CROWS)), cblk(64 * CROWS), blk(256);
+	const dim3 grid = conv_grid<S>(B), cblk(64 * CROWS), blk(256);
 	const int rows = 16;
 	const dim3 wgrid(B * (S / rows));
 	// last layer: weight gradient, then data gradient into the last hidden pre-activation
-	hipLaunchKernelGGL((k_conv_wgrad<CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gw[NL - 1],
+	hipLaunchKernelGGL((k_conv_wgrad<S, CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gw[NL - 1],
 		c->gb[NL - 1], rows, c->dil[NL - 1]);
-	hipLaunchKernelGGL((k_conv_mfma<CIO, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
+	hipLaunchKernelGGL((k_conv_mfma<S, CIO, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
 		nullptr, c->g[NL - 2], nullptr, c->z[NL - 2], 1.0f, c->dil[NL - 1], nullptr);
 	for (int l = NL - 2; l >= 1; l--) {
-		hipLaunchKernelGGL((k_conv_wgrad<CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows,
+		hipLaunchKernelGGL((k_conv_wgrad<S, CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows,
 			c->dil[l]);
-		hipLaunchKernelGGL((k_conv_mfma<CH, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
+		hipLaunchKernelGGL((k_conv_mfma<S, CH, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
 			c->g[l - 1], nullptr, c->z[l - 1], 1.0f, c->dil[l], nullptr);
 	}
-	hipLaunchKernelGGL((k_conv_wgrad<CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows,
+	hipLaunchKernelGGL((k_conv_wgrad<S, CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows,
 		c->dil[0]);
 	HCHK(hipGetLastError());
 	if (hostLoss) {
@@ -673,6 +706,17 @@ int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B,
 		HCHK(hipStreamSynchronize(c->stream));
 	}
 	return 0;
+}
+
+extern "C" {
+
+int qbf_cnn_gradients(qbf_cnn* c, const float* dIn, const float* dTarget, int B, float iscale, float oscale,
+	float* dOut, float* hostLoss)
+{
+	if (!c || !dIn || !dTarget || !dOut || B < 1 || B > c->maxTrain)
+		return cnn_err(-1, "qbf_cnn_gradients: bad argument (1 <= B <= min(maxBatch, 256))");
+	return c->S == 128 ? gradients_t<128>(c, dIn, dTarget, B, iscale, oscale, dOut, hostLoss)
+	                   : gradients_t<64>(c, dIn, dTarget, B, iscale, oscale, dOut, hostLoss);
 }
 
 // one Adam step with the gradients of qbf_cnn_gradients
@@ -695,7 +739,7 @@ int qbf_cnn_adam(qbf_cnn* c, float lr, float b1, float b2, float eps)
 }
 
 // Midpoint collision step with the network in place of the collision operator, on physical
-// states [ncell][4][64][64] (double, device):
+// states [ncell][4][N][N] (double, device):
 //   C(W) ~ oscale * net(iscale * W),  W_half = W + dt/2 C(W),  out = W + dt C(W_half),
 // then the FP64 moment correction of out - W (mask as qbf_moment_correct; 0 = none).
 // Runs on the engine's stream; ncell may exceed maxBatch (processed in chunks).
@@ -703,7 +747,8 @@ int qbf_surrogate_collision_step(qbf_cnn* c, qbf_engine* e, const double* dW, in
 	float oscale, double L, int mask, double* dOut)
 {
 	if (!c || !e || !dW || !dOut || ncell < 1) return cnn_err(-1, "qbf_surrogate_collision_step: bad argument");
-	if (qbf_engine_N(e) != S) return cnn_err(-1, "qbf_surrogate_collision_step: engine grid must be 64");
+	if (qbf_engine_N(e) != c->S) return cnn_err(-1, "qbf_surrogate_collision_step: engine and network grids differ");
+	const int S = c->S;
 	if (dW == dOut) return cnn_err(-1, "qbf_surrogate_collision_step: in-place update not supported");
 	hipStream_t saved = c->stream;
 	c->stream = (hipStream_t)qbf_get_stream(e);
@@ -739,7 +784,8 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 {
 	if (!c || !e || !dW || !hostIdx || !sums || nsample < 1 || nsample > qbf_cnn::RMAX || nsample > c->maxB)
 		return cnn_err(-1, "qbf_surrogate_residual: bad argument (1 <= nsample <= 64 and <= maxBatch)");
-	if (qbf_engine_N(e) != S) return cnn_err(-1, "qbf_surrogate_residual: engine grid must be 64");
+	if (qbf_engine_N(e) != c->S) return cnn_err(-1, "qbf_surrogate_residual: engine and network grids differ");
+	const int S = c->S;
 	for (int j = 0; j < nsample; j++)
 		if (hostIdx[j] < 0 || hostIdx[j] >= ncell) return cnn_err(-1, "qbf_surrogate_residual: cell index out of range");
 	hipStream_t saved = c->stream;
@@ -751,7 +797,7 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 		r = cnn_err(-2, "index upload failed");
 	if (!r) {
 		hipLaunchKernelGGL(k_gather_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, dW, c->sidx, c->sw,
-			nsample);
+			nsample, CREAL * S * S);
 		if (qbf_phys_to_fourier(e, c->sw, CREAL * nsample, c->swf) ||
 			qbf_collision(e, c->swf, nsample, QBF_C, c->swc, nullptr, nullptr, nullptr, 1.0) ||
 			qbf_fourier_to_phys(e, c->swc, CREAL * nsample, c->sw2))

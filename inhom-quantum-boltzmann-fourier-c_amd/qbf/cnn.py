@@ -1,8 +1,8 @@
 """ctypes binding of the CNN surrogate kernels (qbf_cnn_* in include/qbf.h).
 
-The network maps a physical Wigner state of one cell, fp32 planar [4, 64, 64], to a
-[4, 64, 64] field (trained to the collision operator in physical space).  All tensors are
-torch float32 cuda tensors of shape [B, 4, 64, 64]; the kernels run on torch's current stream.
+The network maps a physical Wigner state of one cell, fp32 planar [4, N, N] (N = 64 or 128), to a
+[4, N, N] field (trained to the collision operator in physical space).  All tensors are torch
+float32 cuda tensors of shape [B, 4, N, N]; the kernels run on torch's current stream.
 """
 from __future__ import annotations
 
@@ -12,7 +12,8 @@ import numpy as np
 
 from . import QBFError, lib
 
-NL, CH, CIO, CREAL, S = 6, 64, 32, 4, 64
+NL, CH, CIO, CREAL = 6, 64, 32, 4
+GRIDS = (64, 128)           # image sides the kernels are built for
 
 _sig_done = False
 
@@ -85,13 +86,14 @@ class CNN:
     """Six-layer periodic 3x3 CNN (bf16 weights/activations, fp32 accumulation) on the GPU."""
 
     def __init__(self, max_batch: int, params: np.ndarray | None = None, seed: int = 0, dil=None,
-                 norm: bool = False):
+                 norm: bool = False, N: int = 64):
         import torch
         if not torch.cuda.is_available():
             raise QBFError("no HIP device available")
         self._torch = torch
         h = C.c_void_p()
-        _check(_lib().qbf_cnn_create(S, max_batch, C.byref(h)))
+        _check(_lib().qbf_cnn_create(N, max_batch, C.byref(h)))
+        self.N = N
         self._h = h
         self.max_batch = max_batch
         self.dil = tuple(dil) if dil is not None else (1,) * NL
@@ -128,7 +130,7 @@ class CNN:
         return p
 
     def forward(self, x, iscale: float = 1.0, oscale: float = 1.0, out=None):
-        """out = oscale * net(iscale * x); x float32 cuda [B, 4, 64, 64]."""
+        """out = oscale * net(iscale * x); x float32 cuda [B, 4, N, N]."""
         self._bind()
         out = self._torch.empty_like(x) if out is None else out
         _check(_lib().qbf_cnn_forward(self._h, _ptr(x), x.shape[0], iscale, _ptr(out), oscale))
@@ -150,7 +152,7 @@ class CNN:
 
     # ------------------------------------------------------------------ stepping with the surrogate
     def collision_step(self, engine, W, dt, iscale, oscale, L, mask=0, out=None):
-        """Midpoint collision step of physical states W (float64 cuda [ncell, 4, 64, 64]) with the
+        """Midpoint collision step of physical states W (float64 cuda [ncell, 4, N, N]) with the
         network as collision operator, followed by the moment correction selected by mask."""
         engine._bind()
         out = self._torch.empty_like(W) if out is None else out
@@ -160,7 +162,7 @@ class CNN:
 
     def residual(self, engine, W, cells, iscale, oscale):
         """(sum|C_net - C|, sum|C|, sum|W|) over the given cells of W (float64 cuda
-        [ncell, 4, 64, 64]), with C the engine's exact collision operator."""
+        [ncell, 4, N, N]), with C the engine's exact collision operator."""
         engine._bind()
         idx = np.ascontiguousarray(cells, np.int32)
         r = (C.c_double * 3)()

@@ -1,5 +1,5 @@
-"""CNN surrogate of the collision operator at N = 64: training data, training on the GPU, and
-the stepping policy that keeps it honest.
+"""CNN surrogate of the collision operator (N = 64 or 128): training data, training on the GPU,
+and the stepping policy that keeps it honest.
 
 Training data are physical states of the periodic workload family (qbf.workloads.periodic)
 at several free-streaming times, with the exact collision operator C = Cc + Cd of the FP32
@@ -47,15 +47,16 @@ def make_engine(N=N_SURR, max_cells=512, J=wl.J_ANG, M=wl.M_GL, precision=FP32):
 DILATION = (1, 2, 4, 8, 4, 1)      # receptive field 43 x 43 pixels (an option of the kernels)
 
 
-def make_net(max_batch, seed=0):
+def make_net(max_batch, seed=0, N=N_SURR):
     """The surrogate network: six undilated 3x3 layers without per-cell normalisation.  (Measured
     at N=64, Nx=512 over 40 bare surrogate steps: the dilated and the normalised variants fit the
     operator as well but drift 2-30x more in energy.)"""
-    return CNN(max_batch=max_batch, seed=seed)
+    return CNN(max_batch=max_batch, seed=seed, N=N)
 
 
-def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, rng_seed=0):
-    """(X, Y): float32 cuda tensors [n, 4, 64, 64] of states and their exact collision operator."""
+def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, rng_seed=0, kmax=4):
+    """(X, Y): float32 cuda tensors [n, 4, N, N] of states (the engine's N) and their exact collision
+    operator."""
     import torch
     rng = np.random.default_rng(rng_seed)
     chunk = chunk or engine.max_cells
@@ -63,7 +64,7 @@ def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, r
     for s in seeds:
         for t in times:
             cells = np.sort(rng.choice(Nx, cells_per_state, replace=False))
-            W = wl.periodic(N_SURR, Nx, seed=int(s), t=float(t), cells=cells)
+            W = wl.periodic(engine.N, Nx, seed=int(s), t=float(t), cells=cells, kmax=kmax)
             X.append(torch.from_numpy(W).cuda())
     Xd = torch.cat(X)
     for i in range(0, Xd.shape[0], chunk):

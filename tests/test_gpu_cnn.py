@@ -89,6 +89,36 @@ def test_gradients_match_oracle(cfg):
     assert o == g.size
 
 
+def test_n128_forward_gradients_and_equivariance():
+    """The same network on 128 x 128 images (two 64-pixel row segments per wave): forward and
+    gradients against the oracle, and periodic translation equivariance across the segment seam."""
+    import torch
+    from qbf.cnn import CNN
+    for cfg in CONFIGS:
+        p, x, t = _setup(5, B=2, S=128, norm=cfg["norm"])
+        net = CNN(max_batch=2, params=p, N=128, **cfg)
+        X = torch.from_numpy(x).cuda()
+        out = net.forward(X, iscale=1.5, oscale=0.8).cpu().numpy()
+        ref, _ = R.forward(x, p, iscale=1.5, oscale=0.8, bf16_round=True, **cfg)
+        e = _rel(out, ref)
+        print(f"N=128 {cfg}: forward rel L2 vs bf16 oracle {e:.2e}", flush=True)
+        assert e < 1e-2
+        ys = net.forward(torch.roll(X, shifts=(3, 64 + 5), dims=(2, 3)).contiguous(), iscale=1.5, oscale=0.8)
+        np.testing.assert_allclose(np.roll(out, shift=(3, 64 + 5), axis=(2, 3)), ys.cpu().numpy(), rtol=0,
+                                   atol=1e-6 * np.abs(out).max())
+        loss, _ = net.gradients(X, torch.from_numpy(t).cuda(), iscale=1.2, oscale=0.9)
+        g = net.get_params(grads=True)
+        lref, gref, _ = R.gradients(x, t, p, iscale=1.2, oscale=0.9, bf16_round=True, **cfg)
+        assert abs(loss - lref) <= 5e-3 * lref, (loss, lref)
+        o = 0
+        for l in range(R.NL):
+            for n in (9 * R.cout(l) * R.cin(l), R.cout(l)):
+                e = _rel(g[o:o + n], gref[o:o + n])
+                assert e < 2e-2, (l, n, e)
+                o += n
+        net.close()
+
+
 def test_adam_step_and_padding():
     import torch
     from qbf.cnn import CNN
