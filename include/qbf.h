@@ -109,6 +109,18 @@ int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A
 int qbf_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
 	const double *dBext, double *dWnext);
 
+/* One periodic Strang step (qbf_inhom_step_periodic with these device buffers) captured as a HIP
+ * graph, for callers that replay a fixed step without per-launch host work; no counterpart in the
+ * reference.  The step runs once eagerly first (dOut holds its result), then is captured on the
+ * engine's stream, which must not be the null stream (qbf_set_stream).  Needs a fixed paired-path
+ * choice (qbf_set_real_pairs mode 0 or 2) and no buffer growth during the capture.  Launches replay
+ * on that stream, asynchronously. */
+typedef struct qbf_graph qbf_graph;
+int  qbf_graph_capture_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h,
+	double dt, const double *dBext, double *dOut, qbf_graph **g);
+int  qbf_graph_launch(qbf_graph *g);
+void qbf_graph_destroy(qbf_graph *g);
+
 /* numsteps frames of the periodic problem from dW0 (device), the state kept on the device
  * (src/simulation.c:1288-1301): frame 0 = W0, frame it = qbf_inhom_step_periodic of frame it-1
  * (with the engine's conservation setting).  Frames 0, stride, 2 stride, ... < numsteps are

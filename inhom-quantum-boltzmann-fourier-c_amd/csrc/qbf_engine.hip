@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <stdarg.h>
 #include <string.h>
+#include <deque>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -46,6 +47,10 @@ static int set_err(int code, const char* fmt, ...)
 	} while (0)
 
 extern "C" const char* qbf_last_error(void) { return g_err.c_str(); }
+
+// set while a step is captured into a HIP graph (qbf_graph_capture_inhom_step_periodic): no
+// allocation, host copy, read-back or synchronisation may happen then
+static thread_local bool tl_capturing = false;
 
 // threads per workgroup for a padded P x P grid (P = 2N), and for an unpadded N x N grid
 // XG: exchange rows and columns through a per-workgroup global scratch slot instead of LDS
@@ -168,6 +173,7 @@ struct DevBuf {
 	int ensure(size_t n)
 	{
 		if (n <= bytes) return 0;
+		if (tl_capturing) return set_err(-1, "buffer growth during graph capture");
 		if (p) hipFree(p);
 		p = nullptr;
 		bytes = 0;
@@ -211,7 +217,16 @@ struct EngineT final : public qbf_engine {
 	int n2 = 0; DevBuf i2p1, i2p2, i2pc;
 	int n3 = 0; DevBuf i3p1, i3p2n;
 
-	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, terms, xscr;
+	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, xscr;
+	// Combination-term tables: each distinct table is uploaded once and its device copy reused, so
+	// that evaluations after the first of a configuration issue no host copy and no synchronisation
+	// (which makes the step capturable in a HIP graph).  At most TERM_CACHE tables are kept.
+	struct TermTab {
+		std::vector<CTerm> host;
+		DevBuf dev;
+	};
+	static constexpr size_t TERM_CACHE = 64;
+	std::deque<TermTab> termCache;
 	DevBuf Wmid, WF, WF2, Wt1, Wt2, src3;
 	DevBuf Wpk, WhatP, reim, Ipk;     // paired path: packed spectra, their padded transforms, realness,
 	                                  // packed I1-type results
@@ -221,7 +236,6 @@ struct EngineT final : public qbf_engine {
 	{
 		int r = ensure_twiddles();
 		if (r) return r;
-		r = terms.ensure(sizeof(CTerm) * 4 * 16);
 		return r;
 	}
 
@@ -459,7 +473,7 @@ struct EngineT final : public qbf_engine {
 			r = xs_for(ncell * nout * nchunk, &a.xs, BIG ? big::I3_NI : 1);
 			if (r) return r;
 			hipEvent_t eb = nullptr, ee = nullptr;
-			if (prof && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
+			if (prof && !tl_capturing && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
 			if constexpr (BIG)
 				hipLaunchKernelGGL((k_i3acc_big<R>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			else if (pair)
@@ -558,11 +572,11 @@ struct EngineT final : public qbf_engine {
 		std::vector<CTerm> flat((size_t)nout * maxT);
 		for (int o = 0; o < nout; o++)
 			for (size_t i = 0; i < tl[o].size(); i++) flat[(size_t)o * maxT + i] = tl[o][i];
-		int r = terms.ensure(sizeof(CTerm) * flat.size());
+		const CTerm* dterms = nullptr;
+		int r = term_table(flat, &dterms);
 		if (r) return r;
-		HIPCHK(hipMemcpyAsync(terms.p, flat.data(), sizeof(CTerm) * flat.size(), hipMemcpyHostToDevice, stream));
 		CombineArgs a{};
-		a.terms = terms.template as<CTerm>(); a.maxTerms = maxT; a.nout = nout;
+		a.terms = dterms; a.maxTerms = maxT; a.nout = nout;
 		for (int o = 0; o < nout; o++) { a.nterm[o] = (int)tl[o].size(); a.outComp[o] = outComp[o]; }
 		a.outScale = 1.0; a.out = out; a.base = base; a.Wcb = Bext ? Wcb : nullptr; a.Bext = Bext; a.cf = cf;
 		a.tw256 = g_tw256;
@@ -573,8 +587,45 @@ struct EngineT final : public qbf_engine {
 		else
 			hipLaunchKernelGGL((k_combine<R, N, T, XG>), dim3(ncell * nout), dim3(T), 0, stream, a);
 		HIPCHK(hipGetLastError());
-		// the host term table must outlive the asynchronous copy
-		HIPCHK(hipStreamSynchronize(stream));
+		return 0;
+	}
+
+	static bool same_terms(const std::vector<CTerm>& a, const std::vector<CTerm>& b)
+	{
+		if (a.size() != b.size()) return false;
+		for (size_t i = 0; i < a.size(); i++) {
+			const CTerm &x = a[i], &y = b[i];
+			if (x.X != y.X || x.xcell != y.xcell || x.Y != y.Y || x.ycell != y.ycell || x.ysum != y.ysum ||
+				x.nsum != y.nsum || x.coef != y.coef || x.ax != y.ax || x.bx != y.bx)
+				return false;
+		}
+		return true;
+	}
+
+	// device copy of a combination-term table (uploaded on first use; the table's layout [nout][maxT]
+	// is implied by its size and the caller's maxT, which the kernel receives separately)
+	int term_table(const std::vector<CTerm>& flat, const CTerm** dev)
+	{
+		for (auto& t : termCache)
+			if (same_terms(t.host, flat)) {
+				*dev = t.dev.template as<CTerm>();
+				return 0;
+			}
+		if (tl_capturing) return set_err(-1, "combination table not prepared: run the step once before capturing it");
+		if (termCache.size() >= TERM_CACHE) {
+			HIPCHK(hipStreamSynchronize(stream));     // no kernel reads the table that is dropped
+			termCache.pop_front();
+		}
+		termCache.emplace_back();
+		TermTab& t = termCache.back();
+		t.host = flat;
+		int r = t.dev.ensure(sizeof(CTerm) * flat.size());
+		if (r) {
+			termCache.pop_back();
+			return r;
+		}
+		HIPCHK(hipMemcpyAsync(t.dev.p, t.host.data(), sizeof(CTerm) * flat.size(), hipMemcpyHostToDevice, stream));
+		*dev = t.dev.template as<CTerm>();
 		return 0;
 	}
 
@@ -604,6 +655,8 @@ struct EngineT final : public qbf_engine {
 		// chain is complex-linear in its source, so W_a + i W_b yields out_a + i out_b
 		pairUsed = 0;
 		pairRatio = -1.0;
+		if (!BIG && pairMode == 1 && tl_capturing)
+			return set_err(-1, "graph capture needs a fixed paired-path choice (qbf_set_real_pairs mode 0 or 2)");
 		if (!BIG && pairMode == 1) {
 			r = measure_realness(ncell, &pairRatio);
 			if (r) return r;
@@ -1097,6 +1150,66 @@ int qbf_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L,
 {
 	if (!e || !W || !out) return set_err(-1, "null argument");
 	return e->inhom_step(W, ncell, L, h, dt, Bext, out);
+}
+
+struct qbf_graph {
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipStream_t stream = nullptr;
+};
+
+// One Strang step (qbf_inhom_step_periodic with these buffers) as a HIP graph: the step runs once
+// eagerly (every buffer sized, every combination table uploaded), then is captured on the
+// engine's stream, which must not be the null stream.
+int qbf_graph_capture_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L, double h, double dt,
+	const double* Bext, double* out, qbf_graph** g)
+{
+	if (!e || !W || !out || !g) return set_err(-1, "qbf_graph_capture_inhom_step_periodic: null argument");
+	*g = nullptr;
+	if (!e->stream) return set_err(-1, "qbf_graph_capture_inhom_step_periodic: the null stream cannot be captured");
+	if (e->pairMode == 1 && e->gridN != 128)
+		return set_err(-1, "qbf_graph_capture_inhom_step_periodic: needs a fixed paired-path choice "
+			"(qbf_set_real_pairs mode 0 or 2)");
+	int r = e->inhom_step(W, ncell, L, h, dt, Bext, out);
+	if (r) return r;
+	HIPCHK(hipStreamSynchronize(e->stream));
+	const bool prof = e->prof;
+	e->prof = false;
+	HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+	tl_capturing = true;
+	r = e->inhom_step(W, ncell, L, h, dt, Bext, out);
+	tl_capturing = false;
+	e->prof = prof;
+	hipGraph_t graph = nullptr;
+	const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+	if (r || ce != hipSuccess) {
+		if (graph) hipGraphDestroy(graph);
+		return r ? r : set_err(-2, "hipStreamEndCapture failed: %s", hipGetErrorString(ce));
+	}
+	hipGraphExec_t exec = nullptr;
+	const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+	if (ie != hipSuccess) {
+		hipGraphDestroy(graph);
+		return set_err(-2, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+	}
+	*g = new qbf_graph{graph, exec, e->stream};
+	return 0;
+}
+
+int qbf_graph_launch(qbf_graph* g)
+{
+	if (!g) return set_err(-1, "null graph");
+	HIPCHK(hipGraphLaunch(g->exec, g->stream));
+	return 0;
+}
+
+void qbf_graph_destroy(qbf_graph* g)
+{
+	if (!g) return;
+	hipStreamSynchronize(g->stream);
+	hipGraphExecDestroy(g->exec);
+	hipGraphDestroy(g->graph);
+	delete g;
 }
 
 int qbf_inhom_step_dirichlet(qbf_engine* e, const double* W, int ncell, double L, double h, double dt,

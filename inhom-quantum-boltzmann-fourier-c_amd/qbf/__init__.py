@@ -77,6 +77,9 @@ def lib():
             "qbf_fourier_to_phys": (i, [vp, vp, i, vp]),
             "qbf_transport_periodic": (i, [vp, vp, i, d, d, d, vp]),
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
+            "qbf_graph_capture_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp, C.POINTER(vp)]),
+            "qbf_graph_launch": (i, [vp]),
+            "qbf_graph_destroy": (None, [vp]),
             "qbf_inhom_step_dirichlet": (i, [vp, vp, i, d, d, d, vp, vp]),
             "qbf_transport_dirichlet": (i, [vp, vp, i, d, d, d, vp]),
             "qbf_transport_maxwell": (i, [vp, vp, i, d, d, d, d, vp, vp, vp]),
@@ -104,6 +107,27 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+class Graph:
+    """A captured step (Engine.capture_inhom_step_periodic); keeps its buffers and stream alive."""
+
+    def __init__(self, h, stream, keep):
+        self._h, self.stream, self._keep = h, stream, keep
+
+    def launch(self):
+        _check(lib().qbf_graph_launch(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qbf_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _check(rc: int):
@@ -361,3 +385,18 @@ class Engine:
         out = self._torch.empty_like(W) if out is None else out
         _check(lib().qbf_inhom_step_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
         return out
+
+    def capture_inhom_step_periodic(self, W, L, h, dt, Bext=None, out=None):
+        """HIP graph of one periodic Strang step from the buffer W into the buffer out (returned with
+        the graph).  The step runs once eagerly first.  Capture and replays use a dedicated torch
+        stream (the default stream cannot be captured); Graph.launch() is asynchronous on it."""
+        torch = self._torch
+        out = torch.empty_like(W) if out is None else out
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        g = C.c_void_p()
+        with torch.cuda.stream(stream):
+            self._bind()
+            _check(lib().qbf_graph_capture_inhom_step_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext),
+                                                               _ptr(out), C.byref(g)))
+        return Graph(g, stream, (W, out, Bext)), out
