@@ -109,6 +109,13 @@ int qbf_transport_line_periodic(qbf_engine *e, const double *dU, int n, double A
 int qbf_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h, double dt,
 	const double *dBext, double *dWnext);
 
+/* After qbf_inhom_step_periodic on ncell cells: the state its collision step started from (after
+ * the first half transport; dWin may be NULL) and the collision operator at that state, C(W) plus
+ * the magnetic term, physical [ncell][4][N][N], taken from the step's FP64 midpoint state as
+ * 2 (W_mid - W) / dt (no further evaluation).  For anchoring a surrogate between exact steps;
+ * no counterpart in the reference. */
+int qbf_last_collision(qbf_engine *e, int ncell, double *dWin, double *dC);
+
 /* One periodic Strang step (qbf_inhom_step_periodic with these device buffers) captured as a HIP
  * graph, for callers that replay a fixed step without per-launch host work; no counterpart in the
  * reference.  The step runs once eagerly first (dOut holds its result), then is captured on the
@@ -223,6 +230,12 @@ int    qbf_surrogate_collision_step(qbf_cnn *c, qbf_engine *e, const double *dW,
  * nsample <= 64.  Synchronous. */
 int    qbf_surrogate_residual(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, const int *hostIdx,
 	int nsample, float iscale, float oscale, double sums[3]);
+/* Offset field of the surrogate operator: C(W) ~ oscale * net(iscale * W) + D in the calls above and
+ * below (D: fp32 planar [ncell][4][N][N] device array on the same cells as their states, kept by
+ * pointer until the next call; NULL removes it).  With D = C(W0) - oscale * net(iscale * W0) at an
+ * exact step's state W0 (qbf_last_collision) the surrogate follows the exact operator there and
+ * the network supplies only its change. */
+int    qbf_surrogate_set_offset(qbf_cnn *c, const float *dD);
 /* Strang step as qbf_inhom_step_periodic with the surrogate collision step; ncell <= maxBatch. */
 int    qbf_surrogate_inhom_step_periodic(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double L,
 	double h, double dt, float iscale, float oscale, int mask, double *dOut);

@@ -353,25 +353,32 @@ __global__ void k_to_f32(const double* __restrict__ W, float* __restrict__ x, lo
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) x[i] = (float)W[i];
 }
+// C = Cv (+ D): the network's operator, plus the offset field D when one is set
+__device__ __forceinline__ double op_value(const float* __restrict__ Cv, const float* __restrict__ D, long long i)
+{
+#pragma clang fp contract(off)
+	return D ? (double)Cv[i] + (double)D[i] : (double)Cv[i];
+}
 // x = (float)(W + a C)
-__global__ void k_half_f32(const double* __restrict__ W, const float* __restrict__ Cv, double a, float* __restrict__ x,
-	long long n)
+__global__ void k_half_f32(const double* __restrict__ W, const float* __restrict__ Cv, const float* __restrict__ D,
+	double a, float* __restrict__ x, long long n)
 {
 #pragma clang fp contract(off)
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) x[i] = (float)(W[i] + a * (double)Cv[i]);
+	if (i < n) x[i] = (float)(W[i] + a * op_value(Cv, D, i));
 }
 // out = W + a C
-__global__ void k_update_f64(const double* __restrict__ W, const float* __restrict__ Cv, double a, double* __restrict__ out,
-	long long n)
+__global__ void k_update_f64(const double* __restrict__ W, const float* __restrict__ Cv, const float* __restrict__ D,
+	double a, double* __restrict__ out, long long n)
 {
 #pragma clang fp contract(off)
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) out[i] = W[i] + a * (double)Cv[i];
+	if (i < n) out[i] = W[i] + a * op_value(Cv, D, i);
 }
 
 // gather cells idx[j] of W [*][CELL] into out [nsel][CELL] (CELL = 4 S S values)
-__global__ void k_gather_cells(const double* __restrict__ W, const int* __restrict__ idx, double* __restrict__ out, int nsel,
+template <typename V>
+__global__ void k_gather_cells(const V* __restrict__ W, const int* __restrict__ idx, V* __restrict__ out, int nsel,
 	int CELL)
 {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,15 +387,15 @@ __global__ void k_gather_cells(const double* __restrict__ W, const int* __restri
 	out[i] = W[(size_t)idx[j] * CELL + e];
 }
 
-// res[0] = sum |a - b|, res[1] = sum |b|, res[2] = sum |w|  (a fp32, b and w fp64; one workgroup
-// of 1024 threads)
-__global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ a, const double* __restrict__ b,
-	const double* __restrict__ w, long long n, double* __restrict__ res)
+// res[0] = sum |a (+ D) - b|, res[1] = sum |b|, res[2] = sum |w|  (a and D fp32, b and w fp64;
+// one workgroup of 1024 threads)
+__global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ a, const float* __restrict__ D,
+	const double* __restrict__ b, const double* __restrict__ w, long long n, double* __restrict__ res)
 {
 	__shared__ double sh[3][16];
 	double d = 0, m = 0, s = 0;
 	for (long long i = threadIdx.x; i < n; i += blockDim.x) {
-		d += fabs((double)a[i] - b[i]);
+		d += fabs(op_value(a, D, i) - b[i]);
 		m += fabs(b[i]);
 		s += fabs(w[i]);
 	}
@@ -459,6 +466,9 @@ struct qbf_cnn {
 	static constexpr int RMAX = 64;       // residual check: at most RMAX sampled cells
 	int* sidx = nullptr;                  // sampled cell indices (device)
 	double *swf = nullptr, *swc = nullptr; // Fourier state / exact collision of the sample (complex)
+	// offset field D (fp32 [ncell][4][S][S], the caller's): the surrogate operator is net + D
+	const float* off = nullptr;
+	float* soff = nullptr;                // D on the residual sample
 	double* sres = nullptr;               // three sums of the residual
 	int step = 0;
 	std::vector<void*> allocs;
@@ -518,6 +528,7 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 	c->amp = c->alloc_f(maxBatch);
 	c->sx = c->alloc_f(px * CREAL);
 	c->sc = c->alloc_f(px * CREAL);
+	c->soff = c->alloc_f((size_t)qbf_cnn::RMAX * CREAL * S * S);
 	{
 		void *p1 = nullptr, *p2 = nullptr;
 		if (hipMalloc(&p1, px * CREAL * sizeof(double)) == hipSuccess) c->allocs.push_back(p1);
@@ -539,7 +550,7 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		c->swc = (double*)p3;
 		c->sres = (double*)p4;
 	}
-	if (!c->loss || !c->amp || !c->sx || !c->sc || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
+	if (!c->loss || !c->amp || !c->sx || !c->sc || !c->soff || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
 		delete c;
 		return cnn_err(-3, "qbf_cnn_create: out of device memory");
 	}
@@ -761,10 +772,11 @@ int qbf_surrogate_collision_step(qbf_cnn* c, qbf_engine* e, const double* dW, in
 		hipLaunchKernelGGL(k_to_f32, g, b, 0, c->stream, dW + off, c->sx, n);
 		r = run_forward(c, c->sx, nb, iscale, c->sc, oscale, false);
 		if (r) break;
-		hipLaunchKernelGGL(k_half_f32, g, b, 0, c->stream, dW + off, c->sc, 0.5 * dt, c->sx, n);
+		const float* D = c->off ? c->off + off : nullptr;
+		hipLaunchKernelGGL(k_half_f32, g, b, 0, c->stream, dW + off, c->sc, D, 0.5 * dt, c->sx, n);
 		r = run_forward(c, c->sx, nb, iscale, c->sc, oscale, false);
 		if (r) break;
-		hipLaunchKernelGGL(k_update_f64, g, b, 0, c->stream, dW + off, c->sc, dt, dOut + off, n);
+		hipLaunchKernelGGL(k_update_f64, g, b, 0, c->stream, dW + off, c->sc, D, dt, dOut + off, n);
 		if (hipGetLastError() != hipSuccess) r = cnn_err(-2, "surrogate step launch failed");
 	}
 	if (!r && mask && qbf_moment_correct(e, dW, dOut, ncell, L, mask))
@@ -796,8 +808,11 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 	if (hipMemcpyAsync(c->sidx, hostIdx, nsample * sizeof(int), hipMemcpyHostToDevice, c->stream) != hipSuccess)
 		r = cnn_err(-2, "index upload failed");
 	if (!r) {
-		hipLaunchKernelGGL(k_gather_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, dW, c->sidx, c->sw,
-			nsample, CREAL * S * S);
+		hipLaunchKernelGGL(k_gather_cells<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, dW, c->sidx,
+			c->sw, nsample, CREAL * S * S);
+		if (c->off)
+			hipLaunchKernelGGL(k_gather_cells<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->off,
+				c->sidx, c->soff, nsample, CREAL * S * S);
 		if (qbf_phys_to_fourier(e, c->sw, CREAL * nsample, c->swf) ||
 			qbf_collision(e, c->swf, nsample, QBF_C, c->swc, nullptr, nullptr, nullptr, 1.0) ||
 			qbf_fourier_to_phys(e, c->swc, CREAL * nsample, c->sw2))
@@ -808,7 +823,8 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 		r = run_forward(c, c->sx, nsample, iscale, c->sc, oscale, false);
 	}
 	if (!r) {
-		hipLaunchKernelGGL(k_l1_residual, dim3(1), dim3(1024), 0, c->stream, c->sc, c->sw2, c->sw, n, c->sres);
+		hipLaunchKernelGGL(k_l1_residual, dim3(1), dim3(1024), 0, c->stream, c->sc, c->off ? c->soff : nullptr, c->sw2,
+			c->sw, n, c->sres);
 		if (hipMemcpyAsync(h, c->sres, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
 			hipStreamSynchronize(c->stream) != hipSuccess)
 			r = cnn_err(-2, "residual read-back failed");
@@ -817,6 +833,16 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 	if (!r)
 		for (int k = 0; k < 3; k++) sums[k] = h[k];
 	return r;
+}
+
+// Offset field of the surrogate operator: C(W) ~ oscale * net(iscale * W) + D (D fp32, planar
+// [ncell][4][N][N] device array, the same cells as the states passed to the surrogate calls; the
+// pointer is kept until the next call; NULL: no offset).
+int qbf_surrogate_set_offset(qbf_cnn* c, const float* dD)
+{
+	if (!c) return cnn_err(-1, "qbf_surrogate_set_offset: null network");
+	c->off = dD;
+	return 0;
 }
 
 // Strang step of the periodic inhomogeneous problem with the surrogate collision step:

@@ -125,6 +125,10 @@ struct qbf_engine {
 	virtual int fourier2phys(const double2* in, int ngrid, double* out) = 0;
 	virtual int transport(const double* W, int ncell, double L, double h, double dt, double* out) = 0;
 	virtual int inhom_step(const double* W, int ncell, double L, double h, double dt, const double* Bext, double* out) = 0;
+	// the last periodic Strang step: its cell count and dt (0 cells: none since the last other step)
+	int lastCells = 0;
+	double lastDt = 0.0;
+	virtual int last_collision(int ncell, double* Win, double* C) = 0;
 	virtual int inhom_step_dirichlet(const double* W, int ncell, double L, double h, double dt, const double* Bext,
 		double* out) = 0;
 	int transport_dirichlet(const double* W, int ncell, double L, double h, double dt, double* out)
@@ -753,6 +757,7 @@ struct EngineT final : public qbf_engine {
 
 	int collision_step(const double2* W, int ncell, double dt, const double* Bext, double2* Wnext) override
 	{
+		lastCells = 0;      // Wmid is rewritten (inhom_step sets it again afterwards)
 		int r = Wmid.ensure(sizeof(double2) * 4 * NN * (size_t)ncell);
 		if (r) return r;
 		double2* Wm = Wmid.template as<double2>();
@@ -896,7 +901,29 @@ struct EngineT final : public qbf_engine {
 		if (!r) r = fourier2phys(WF2.template as<double2>(), 4 * ncell, Wt2.template as<double>());
 		if (!r && momentMask) r = moment_correct(Wt1.template as<double>(), Wt2.template as<double>(), ncell, L, momentMask);
 		if (!r) r = transport(Wt2.template as<double>(), ncell, L, h, 0.5 * dt, out);
+		lastCells = r ? 0 : ncell;
+		lastDt = dt;
 		return r;
+	}
+
+	// The state the last periodic step's collision started from (after the first half transport) and
+	// the collision operator there (with the magnetic term), in physical space: C = 2 (W_mid - W) / dt
+	// from the step's FP64 midpoint state, so no further evaluation of the operator
+	DevBuf Cf;
+	int last_collision(int ncell, double* Win, double* C) override
+	{
+		if (lastCells < 1 || ncell != lastCells)
+			return set_err(-1, "no periodic step on %d cells to take the collision operator from", ncell);
+		const size_t nphys = sizeof(double) * 4 * NN * (size_t)ncell;
+		if (Win) HIPCHK(hipMemcpyAsync(Win, Wt1.p, nphys, hipMemcpyDeviceToDevice, stream));
+		if (!C) return 0;
+		int r = Cf.ensure(2 * nphys);
+		if (r) return r;
+		const long long n = 4LL * NN * ncell;
+		hipLaunchKernelGGL(k_scaled_diff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+			Wmid.template as<double2>(), WF.template as<double2>(), 2.0 / lastDt, Cf.template as<double2>(), n);
+		HIPCHK(hipGetLastError());
+		return fourier2phys(Cf.template as<double2>(), 4 * ncell, C);
 	}
 
 	// Dirichlet step (src/simulation.c:562-830): half transport with Dirichlet ends, collision on
@@ -1150,6 +1177,12 @@ int qbf_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L,
 {
 	if (!e || !W || !out) return set_err(-1, "null argument");
 	return e->inhom_step(W, ncell, L, h, dt, Bext, out);
+}
+
+int qbf_last_collision(qbf_engine* e, int ncell, double* dWin, double* dC)
+{
+	if (!e) return set_err(-1, "null engine");
+	return e->last_collision(ncell, dWin, dC);
 }
 
 struct qbf_graph {

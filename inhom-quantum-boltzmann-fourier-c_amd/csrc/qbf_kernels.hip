@@ -410,6 +410,18 @@ __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 }
 
 // ---------------------------------------------------------------------------------------------
+// out = s (a - b) over n complex values: the collision operator of a midpoint step recovered from
+// its midpoint state a = b + dt/2 C(b) and start state b, with s = 2 / dt
+__global__ void k_scaled_diff(const double2* __restrict__ a, const double2* __restrict__ b, double s,
+	double2* __restrict__ out, long long n)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const double2 x = a[i], y = b[i];
+	out[i] = make_double2(s * (x.x - y.x), s * (x.y - y.y));
+}
+
+// ---------------------------------------------------------------------------------------------
 // N x N transforms of the state (src/simulation.c:224-283): physical real -> Fourier / N^2
 // (full complex FFT of the real data), and Fourier -> unnormalised backward FFT -> real part.
 // XG: exchange through the per-workgroup slot xs + blockIdx.x * Xpose::SIZE (N = 128)

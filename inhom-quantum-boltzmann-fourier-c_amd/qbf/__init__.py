@@ -79,6 +79,7 @@ def lib():
             "qbf_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp]),
             "qbf_graph_capture_inhom_step_periodic": (i, [vp, vp, i, d, d, d, vp, vp, C.POINTER(vp)]),
             "qbf_graph_launch": (i, [vp]),
+            "qbf_last_collision": (i, [vp, i, vp, vp]),
             "qbf_graph_destroy": (None, [vp]),
             "qbf_inhom_step_dirichlet": (i, [vp, vp, i, d, d, d, vp, vp]),
             "qbf_transport_dirichlet": (i, [vp, vp, i, d, d, d, vp]),
@@ -385,6 +386,17 @@ class Engine:
         out = self._torch.empty_like(W) if out is None else out
         _check(lib().qbf_inhom_step_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
         return out
+
+    def last_collision(self, ncell, with_state=True):
+        """After inhom_step_periodic on ncell cells: (the state its collision step started from, or
+        None; the collision operator there), float64 cuda [ncell, 4, N, N], from the step's FP64
+        midpoint (no further evaluation)."""
+        self._bind()
+        shape = (ncell, 4, self.N, self.N)
+        Win = self._torch.empty(shape, dtype=self._torch.float64, device="cuda") if with_state else None
+        Cp = self._torch.empty(shape, dtype=self._torch.float64, device="cuda")
+        _check(lib().qbf_last_collision(self._h, ncell, _ptr(Win), _ptr(Cp)))
+        return Win, Cp
 
     def capture_inhom_step_periodic(self, W, L, h, dt, Bext=None, out=None):
         """HIP graph of one periodic Strang step from the buffer W into the buffer out (returned with

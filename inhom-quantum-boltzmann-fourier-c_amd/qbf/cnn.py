@@ -40,6 +40,7 @@ def _lib():
             "qbf_surrogate_residual": (i, [vp, vp, vp, i, C.POINTER(i), i, f, f, C.POINTER(C.c_double)]),
             "qbf_surrogate_inhom_step_periodic": (i, [vp, vp, vp, i, C.c_double, C.c_double, C.c_double, f, f, i,
                                                       vp]),
+            "qbf_surrogate_set_offset": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -151,6 +152,12 @@ class CNN:
         _check(_lib().qbf_cnn_adam(self._h, lr, b1, b2, eps))
 
     # ------------------------------------------------------------------ stepping with the surrogate
+    def set_offset(self, D=None):
+        """Offset field of the surrogate operator, C(W) ~ net(W) + D (float32 cuda [ncell, 4, N, N] on
+        the cells of the states passed to the stepping calls), or None."""
+        self._offset = D
+        _check(_lib().qbf_surrogate_set_offset(self._h, None if D is None else _ptr(D)))
+
     def collision_step(self, engine, W, dt, iscale, oscale, L, mask=0, out=None):
         """Midpoint collision step of physical states W (float64 cuda [ncell, 4, N, N]) with the
         network as collision operator, followed by the moment correction selected by mask."""

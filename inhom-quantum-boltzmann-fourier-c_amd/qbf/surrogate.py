@@ -15,7 +15,12 @@ Stepping policy (SurrogateStepper.step), per Strang step of the periodic problem
     and above `tol` the step falls back to the exact step.  (A relative error of the operator
     itself, sum|C_net - C| / sum|C|, is meaningless near local equilibrium, where C -> 0.)
   * a surrogate step is the midpoint collision step with the network, followed by the FP64
-    conservation correction (mass, spin, momentum, energy), between two exact half transports.
+    conservation correction (mass, spin, momentum, energy), between two exact half transports;
+  * anchoring (the default): after every exact step the operator is anchored to the exact one,
+        C(W) ~ C(W0) + net(W) - net(W0),
+    with W0 the state the exact step's collision started from and C(W0) taken from that step
+    (Engine.last_collision: no extra evaluation); the network then supplies only the change of
+    the operator since the exact step, and its systematic error cancels.
 """
 from __future__ import annotations
 
@@ -122,12 +127,13 @@ class SurrogateStepper:
     """Strang steps of the periodic problem with the surrogate collision step and its safeguards."""
 
     def __init__(self, engine, net, iscale, oscale, L, h, dt, exact_every=10, sample_frac=0.02, tol=1e-3,
-                 mask=CONS_ALL, check=True, seed=0):
+                 mask=CONS_ALL, check=True, seed=0, anchor=True):
         self.e, self.net = engine, net
         self.iscale, self.oscale = iscale, oscale
         self.L, self.h, self.dt = L, h, dt
         self.exact_every, self.sample_frac, self.tol = exact_every, sample_frac, tol
-        self.mask, self.check = mask, check
+        self.mask, self.check, self.anchor = mask, check, anchor
+        net.set_offset(None)
         self.rng = np.random.default_rng(seed)
         self.nstep = 0
         self.counts = {"exact": 0, "surrogate": 0, "fallback": 0}
@@ -136,7 +142,12 @@ class SurrogateStepper:
 
     def _exact(self, W, out):
         self.e.set_moment_correction(self.mask)
-        return self.e.inhom_step_periodic(W, self.L, self.h, self.dt, out=out)
+        out = self.e.inhom_step_periodic(W, self.L, self.h, self.dt, out=out)
+        if self.anchor:
+            W0, C0 = self.e.last_collision(W.shape[0])
+            N0 = self.net.forward(W0.float().contiguous(), self.iscale, self.oscale)
+            self.net.set_offset((C0 - N0.double()).float().contiguous())
+        return out
 
     def step(self, W, out=None):
         ncell = W.shape[0]
