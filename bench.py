@@ -122,6 +122,7 @@ def main():
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(N, J, M, cores, args.seed)
 
+    print("  loading torch and the engine", file=sys.stderr, flush=True)
     import numpy as np
     import torch
 
@@ -135,11 +136,16 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    def progress(msg):
+        print(f"  {msg}", file=sys.stderr, flush=True)
+
     W0 = torch.from_numpy(wl.periodic(N, Nx, seed=args.seed, kmax=args.kmax)).to(dev)
+    progress("initial state generated")
     w0 = W0.cpu().numpy()
     h = 1.0 / Nx
     dt = wl.cfl_dt(N, Nx)
     tabs = qbf.quadrature(N, J, wl.L_BOX, wl.R_BALL, M)
+    progress("quadrature tables built")
     evals_per_step = 2 * Nx
     P = 2 * N
     flop_per_k = 2 * 5 * P * P * np.log2(P * P)     # IFFT2 + FFT2 of a P x P complex grid
@@ -240,6 +246,7 @@ def main():
         t0 = time.perf_counter()
         X, Y = S.training_set(e32, seeds=range(100, 116), times=np.linspace(0.0, 0.04, 9),
                               cells_per_state=args.train_cells, Nx=Nx, kmax=args.kmax)
+        progress(f"training set: {X.shape[0]} exact pairs")
         Xv, Yv = S.training_set(e32, seeds=[args.seed], times=[0.0, 0.03], cells_per_state=32, Nx=Nx, rng_seed=1,
                                 kmax=args.kmax)
         isc, osc = S.scales(X, Y, norm=False)
@@ -248,7 +255,7 @@ def main():
         t_data = time.perf_counter() - t0
         net = S.make_net(Nx, seed=0, N=N)
         t0 = time.perf_counter()
-        hist = S.train(net, X, Y, isc, osc, steps=args.train_steps)
+        hist = S.train(net, X, Y, isc, osc, steps=args.train_steps, log_every=500)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         val = S.rel_l1(net, Xv, Yv, isc, osc)

@@ -25,6 +25,7 @@ Stepping policy (SurrogateStepper.step), per Strang step of the periodic problem
 from __future__ import annotations
 
 import math
+import sys
 import time
 
 import numpy as np
@@ -117,7 +118,7 @@ def train(net, X, Y, iscale, oscale, steps=3000, batch=32, lr=2e-3, lr_min=2e-5,
         net.adam(a)
         hist.append(loss)
         if log_every and it % log_every == 0:
-            print(f"  train step {it}: loss {loss:.3e} lr {a:.2e}", flush=True)
+            print(f"  train step {it}: loss {loss:.3e} lr {a:.2e}", file=sys.stderr, flush=True)
         if not math.isfinite(loss):
             raise QBFError(f"training diverged at step {it}")
     return hist
