@@ -179,7 +179,10 @@ def main():
             cur = 1 - cur
             if i in keep_at:
                 kept[i] = bufs[cur].clone()
-            # long runs report their progress (stderr): every 25 steps, or after steps of > 10 s
+            # long runs report their progress (stderr): every 25 steps, or after steps of > 10 s (short
+            # runs wait for each step, whose kernels are queued asynchronously, to see how long it took)
+            if k < 50:
+                torch.cuda.synchronize()
             if (k >= 50 and i % 25 == 0) or time.perf_counter() - t_last > 10:
                 torch.cuda.synchronize()
                 print(f"  {i}/{k} steps", file=sys.stderr, flush=True)
