@@ -1350,7 +1350,6 @@ int qbf_fd_distance(qbf_engine* e, const double* dW, int ncell, double L, const 
 {
 	if (!e || !dW || !dWFD || !dOut || ncell < 1 || (fd_stride != 0 && fd_stride != 1))
 		return set_err(-1, "qbf_fd_distance: bad argument");
-	if (e->gridN > 64) return set_err(-1, "qbf_fd_distance: N = %d above 64", e->gridN);
 	hipLaunchKernelGGL(k_fd_distance, dim3(ncell), dim3(256), 0, e->stream, dW, dWFD, fd_stride, e->gridN, L, dOut);
 	HIPCHK(hipGetLastError());
 	return 0;

@@ -1065,7 +1065,7 @@ __device__ double relentropy2(double a, double b1, double b2, double b3, double 
 // out[cell][0] = sum_j ||W_j - WFD_j||_1 dv with the matrix 1-norm of the N x N grid (maximum over
 // the slow index l of sum_k |.|, as MATLAB's norm(A, 1) on the [k, l] matrix), out[cell][1] =
 // sum_v S(W(v) || WFD(v)) dv over the velocities with WFD_0(v) > 1e-6.  fd_stride 0: one state WFD
-// for every cell (the global equilibrium); 1: one per cell.  N <= 64.
+// for every cell (the global equilibrium); 1: one per cell.
 __global__ __launch_bounds__(256) void k_fd_distance(const double* __restrict__ W, const double* __restrict__ WFD,
 	int fd_stride, int N, double L, double* __restrict__ out)
 {
@@ -1074,13 +1074,16 @@ __global__ __launch_bounds__(256) void k_fd_distance(const double* __restrict__ 
 	const double* w = W + (size_t)blockIdx.x * 4 * NN;
 	const double* f = WFD + (size_t)blockIdx.x * fd_stride * 4 * NN;
 	const double sc = (2 * L) / N, dv = sc * sc;
-	// one wave per component, one lane per column l
-	const int j = threadIdx.x >> 6, l = threadIdx.x & 63;
+	// one wave per component, a lane per column l (and l + 64, ... for N > 64)
+	const int j = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	double col = 0;
-	if (l < N)
-		for (int k = 0; k < N; k++) col += fabs(w[j * NN + k + N * l] - f[j * NN + k + N * l]);
+	for (int l = lane; l < N; l += 64) {
+		double c = 0;
+		for (int k = 0; k < N; k++) c += fabs(w[j * NN + k + N * l] - f[j * NN + k + N * l]);
+		col = fmax(col, c);
+	}
 	for (int o = 32; o > 0; o >>= 1) col = fmax(col, __shfl_xor(col, o));
-	double dist = (l == 0) ? col : 0.0;
+	double dist = (lane == 0) ? col : 0.0;
 	dist = block_sum_256(dist, sh);
 	double kl = 0;
 	for (int e = threadIdx.x; e < NN; e += blockDim.x)

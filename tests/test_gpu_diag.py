@@ -8,7 +8,7 @@ from oracle import diag_ref as D
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N", [16, 32, 64])
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
 def test_cell_diagnostics_vs_oracle(N):
     import torch
     import qbf
@@ -24,7 +24,7 @@ def test_cell_diagnostics_vs_oracle(N):
     assert err < 1e-12
 
 
-@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("N", [32, 64, 128])
 def test_fermi_dirac_fit_vs_oracle(N):
     """qbf_fermi_dirac against asymptoticFD.m as restated in the oracle (least squares on the three
     moment equations; the kernel solves the equivalent one-dimensional equation in beta)."""
@@ -50,12 +50,13 @@ def test_fermi_dirac_fit_vs_oracle(N):
     assert prm[-1, 11] == 1 and not np.any(wfd[-1])
 
 
-def test_fd_distance_and_cell_average_vs_oracle():
+@pytest.mark.parametrize("N", [32, 128])
+def test_fd_distance_and_cell_average_vs_oracle(N):
     """qbf_fd_distance (per-cell and global equilibrium) and qbf_cell_average against the oracle,
     on the oracle's own Fermi-Dirac states."""
     import torch
     import qbf
-    N, L, n = 32, 6.0, 4
+    L, n = 6.0, 4
     rng = np.random.default_rng(9)
     W = rand_state(rng, n, N, L)
     e = qbf.Engine(N, qbf.FP64, max_cells=n)
