@@ -112,3 +112,25 @@ def test_n128_has_no_paired_path():
             e.set_real_pairs(1, 1e-5)
     finally:
         e.close()
+
+
+def test_n128_boundary_steps_vs_oracle(e64):
+    """Dirichlet and Maxwell Strang steps at N = 128 (transport kernels take N at run time; the
+    collision of the interior / all cells runs the N = 128 kernels) against the oracle."""
+    import torch
+    q = O.Quad(N, 4, L, R, 2)
+    e64.set_quadrature(q.I1, q.I2, q.I3, q.I4)
+    rng = np.random.default_rng(21)
+    nx, h, dt, lam = 3, 0.08, 0.002, 0.4
+    W = rand_state(rng, nx, N, L)
+    WL, WR = rand_state(rng, 2, N, L)
+    B = rng.standard_normal(3 * nx) * 0.3
+    print("N=128 Dirichlet and Maxwell steps on 3 cells", flush=True)
+    got = host(e64.inhom_step_dirichlet(dev(W), L, h, dt, dev(B)))
+    err = l1rel(got, O.inhom_step_dirichlet(q, W, h, dt, B))
+    print(f"Dirichlet {err:.2e}", flush=True)
+    assert err < TOL64
+    got = host(e64.inhom_step_maxwell(dev(W), L, h, dt, lam, dev(WL), dev(WR), dev(B)))
+    err = l1rel(got, O.inhom_step_maxwell(q, W, h, dt, lam, WL, WR, B))
+    print(f"Maxwell {err:.2e}")
+    assert err < TOL64
