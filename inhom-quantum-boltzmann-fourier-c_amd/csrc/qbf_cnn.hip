@@ -222,14 +222,15 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 //   dW[t][co][ci] += sum_p dz[p][co] * x[p + dil off_t][ci],  db[co] += sum_p dz[p][co]
 // One workgroup per (image, block of ROWS rows); thread (co, 16-channel ci block) keeps
 // 9 x 16 accumulators; the three rows of x that a row of dz meets and that row of dz are
-// staged in LDS (periodic in x).
+// staged in LDS (periodic in x; the x rows with a pitch of CI + 4 floats, so that a thread reads
+// its 16 channels as four aligned 16-byte words).
 template <int S, int CI, int CO>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dz,
 	float* __restrict__ dW, float* __restrict__ db, int rows, int dil)
 {
 	constexpr int NCB = CI / 16;                 // ci blocks of 16
 	constexpr int NTH = CO * NCB;                // active threads
-	__shared__ float xs[3][S][CI + 1];
+	__shared__ __attribute__((aligned(16))) float xs[3][S][CI + 4];
 	__shared__ float zs[S][CO + 1];
 	const int b = blockIdx.x / (S / rows);
 	const int y0 = (blockIdx.x % (S / rows)) * rows;
