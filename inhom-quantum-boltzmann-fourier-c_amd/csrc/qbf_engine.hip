@@ -447,7 +447,10 @@ struct EngineT final : public qbf_engine {
 		if (n3 <= 0) return set_err(-1, "I3 table not set");
 		if (BIG && pair) return set_err(-1, "no paired path at N = 128");
 		const int nq = n3;
-		int nchunk = (1024 + ncell * nout - 1) / (ncell * nout);
+		// term ranges: about 1024 workgroups when there are few cells, a power of two so that the
+		// ranges of a power-of-two term count are equal and the workgroups fill whole waves
+		int nchunk = 1;
+		while (nchunk * ncell * nout < 1024) nchunk *= 2;
 		if (nchunk > nq) nchunk = nq;
 		if (nchunk < 1) nchunk = 1;
 		const size_t perQ = (size_t)ncell * nF * GB * (XG ? 2 : 1);
@@ -633,6 +636,17 @@ struct EngineT final : public qbf_engine {
 		return 0;
 	}
 
+	// buf[o][c] (+)= ..., o < nouter, c < nch grids: buf[o][0] = sum_c buf[o][c]
+	int sum_chunks(void* buf, long long nouter, int nch)
+	{
+		if (nch <= 1) return 0;
+		const long long n = nouter * PP;
+		hipLaunchKernelGGL((k_sum_chunks<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (C*)buf, nch,
+			nouter, (long long)PP);
+		HIPCHK(hipGetLastError());
+		return 0;
+	}
+
 	static CTerm term(const void* X, long long xcell, const void* Y, long long ycell, double coef,
 		double ax = 1, double bx = 0, int nsum = 1, long long ysum = 0)
 	{
@@ -712,6 +726,13 @@ struct EngineT final : public qbf_engine {
 			}
 			if (r) return r;
 		}
+		// the partial sums over term ranges are added here over the whole grid, so that k_combine
+		// reads one grid per term (added in the order k_combine used: the result is unchanged)
+		if (cc) r = sum_chunks(Ahat.p, 3LL * ncell, nchA);
+		if (!r && cd) r = sum_chunks(Bhat.p, 4LL * ncell, nchB);
+		if (!r && cd) r = sum_chunks(Sbuf.p, 4LL * ncell, nchS);
+		if (!r && cd) r = sum_chunks(acc3.p, 4LL * ncell, nchunk);
+		if (r) return r;
 		const double p2 = 1.0 / ((double)P * P), p4 = p2 * p2;
 		const double eta2[4] = {2, -2, -2, -2};
 		const C* Wh = What.template as<C>();
@@ -722,13 +743,13 @@ struct EngineT final : public qbf_engine {
 			const long long a3 = 3LL * nchA * PP;
 			auto As = [&](int s) { return (const void*)(A + (size_t)s * nchA * PP); };
 			// Cc_1 = I1(f,W2,W3) - I1(f,W3,W2), cyclic (src/collision.c:231-253); Ahat slot s <-> g = s+1
-			// (each slot: nchA partial sums over term ranges)
-			tl[1].push_back(term(Wslot(3), c4, As(1), a3, p4, 1, 0, nchA, PP));
-			tl[1].push_back(term(Wslot(2), c4, As(2), a3, -p4, 1, 0, nchA, PP));
-			tl[2].push_back(term(Wslot(1), c4, As(2), a3, p4, 1, 0, nchA, PP));
-			tl[2].push_back(term(Wslot(3), c4, As(0), a3, -p4, 1, 0, nchA, PP));
-			tl[3].push_back(term(Wslot(2), c4, As(0), a3, p4, 1, 0, nchA, PP));
-			tl[3].push_back(term(Wslot(1), c4, As(1), a3, -p4, 1, 0, nchA, PP));
+			// (each slot: nchA partial sums over term ranges, added into the first)
+			tl[1].push_back(term(Wslot(3), c4, As(1), a3, p4, 1, 0, 1, PP));
+			tl[1].push_back(term(Wslot(2), c4, As(2), a3, -p4, 1, 0, 1, PP));
+			tl[2].push_back(term(Wslot(1), c4, As(2), a3, p4, 1, 0, 1, PP));
+			tl[2].push_back(term(Wslot(3), c4, As(0), a3, -p4, 1, 0, 1, PP));
+			tl[3].push_back(term(Wslot(2), c4, As(0), a3, p4, 1, 0, 1, PP));
+			tl[3].push_back(term(Wslot(1), c4, As(1), a3, -p4, 1, 0, 1, PP));
 		}
 		if (cd) {
 			const C* B = Bhat.template as<C>();
@@ -738,18 +759,18 @@ struct EngineT final : public qbf_engine {
 			auto Bs = [&](int s) { return (const void*)(B + (size_t)s * nchB * PP); };
 			for (int j = 0; j < 4; j++) {
 				// I4 terms (src/collision.c:137-167)
-				tl[j].push_back(term(Wslot(0), c4, Bs(j), b4, eta2[j] * p4, 1, 0, nchB, PP));
-				tl[j].push_back(term(Wslot(j), c4, Bs(0), b4, 2 * p4, 1, 0, nchB, PP));
+				tl[j].push_back(term(Wslot(0), c4, Bs(j), b4, eta2[j] * p4, 1, 0, 1, PP));
+				tl[j].push_back(term(Wslot(j), c4, Bs(0), b4, 2 * p4, 1, 0, 1, PP));
 				if (j == 0)
 					for (int jj = 0; jj < 4; jj++)
-						tl[j].push_back(term(Wslot(jj), c4, Bs(jj), b4, -2 * p4, 1, 0, nchB, PP));
+						tl[j].push_back(term(Wslot(jj), c4, Bs(jj), b4, -2 * p4, 1, 0, 1, PP));
 				// I3 terms: -eta2_j / P^4 * sum_chunks acc_j (src/collision.c:110-121)
 				tl[j].push_back(term(nullptr, 0, A3 + (size_t)j * nchunk * PP, 4LL * nchunk * PP, -eta2[j] * p4,
-					1, 0, nchunk, PP));
+					1, 0, 1, PP));
 				// I2 terms: eta2_i / P^2 * (delta_j0 - What_j) .* S_i (src/collision.c:82-106)
 				for (int i = 0; i < 4; i++)
 					tl[j].push_back(term(Wslot(j), c4, S + (size_t)i * nchS * PP, s4, eta2[i] * p2, -1.0,
-						j == 0 ? 1.0 : 0.0, nchS, PP));
+						j == 0 ? 1.0 : 0.0, 1, PP));
 			}
 		}
 		return launch_combine(tl, 4, all4, ncell, out, base, Wcb, Bext, cf);

@@ -410,6 +410,21 @@ __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Partial sums over term ranges, buf[o][c][e] (c < nch): buf[o][0][e] = sum_c buf[o][c][e], added in
+// chunk order (the order k_combine adds them in when it is given the chunks)
+template <typename R>
+__global__ void k_sum_chunks(cx<R>* __restrict__ buf, int nch, long long nouter, long long PPe)
+{
+	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nouter * PPe) return;
+	const long long o = i / PPe, e = i % PPe;
+	cx<R>* p = buf + o * nch * PPe + e;
+	cx<R> s = p[0];
+	for (int c = 1; c < nch; c++) s = cadd<R>(s, p[(long long)c * PPe]);
+	p[0] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
 // out = s (a - b) over n complex values: the collision operator of a midpoint step recovered from
 // its midpoint state a = b + dt/2 C(b) and start state b, with s = 2 / dt
 __global__ void k_scaled_diff(const double2* __restrict__ a, const double2* __restrict__ b, double s,
