@@ -1,7 +1,7 @@
 """Seeded synthetic initial states for the benchmark configurations.
 
-All generators are deterministic functions of (seed, sizes), so the host, the GPU box
-and the oracle see identical inputs.  States are physical Wigner states
+All generators are deterministic functions of (seed, sizes), so every machine and the
+oracle see identical inputs.  States are physical Wigner states
 [ncell, 4, N, N] (wignerV_t per cell): w0 is the trace part, (w1, w2, w3) the Pauli
 vector, and every state has |(w1,w2,w3)| = 0.8 w0, so the eigenvalues w0 (1 +- 0.8)
 stay in [0, 1] when max w0 <= 0.5.
