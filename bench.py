@@ -47,6 +47,11 @@ REF_FILE = os.path.join(ROOT, "tests", "golden", "periodic64_fp64_ref.npz")
 # HBM bytes per k_i3acc launch at N=64, Nx=512 (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated):
 # complex kernel (8 q x 512 cells x 16 applications) and paired kernel (16 q x 512 cells x 8)
 I3_TRAFFIC_BYTES = {False: 10.34e9, True: 5.18e9}
+# N=128 (k_i3acc_big<float>): HBM bytes per K-operator application, from the same two copy-calibrated
+# passes over `bench.py --N 128 --Nx 64 --no-cpu-baseline --paths mixed --mixed-steps 1 --warmup 0
+# --copy-calibration`: FETCH_SIZE 9.84e6 KiB x 2.000 + WRITE_SIZE 1.052e7 KiB x 1.000 per launch of 8192
+# applications (the scratch grids of the line passes go through HBM)
+I3_TRAFFIC_PER_APP_N128 = 3.77e6
 
 
 def cpu_baseline(N, J, M, cores, seed):
@@ -348,7 +353,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": i3_kernel,
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp32"], 4),
-                     "traffic": I3_TRAFFIC_BYTES[bool(used)] if N == 64 else None,
+                     "traffic": (I3_TRAFFIC_BYTES[bool(used)] if N == 64 else
+                                 round(I3_TRAFFIC_PER_APP_N128 * kapps / max(1, nl)) if N == 128 else None),
                      "launches": nl, "avg_launch_ms": round(kms / max(1, nl), 4),
                      "kernel_share_of_timed": round(kms / 1e3 / twall, 4)},
         "cpu_baseline": cpu,
