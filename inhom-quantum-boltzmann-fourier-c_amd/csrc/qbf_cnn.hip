@@ -477,16 +477,16 @@ struct qbf_cnn {
 	{
 		void* p = nullptr;
 		if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) return nullptr;
-		hipMemset(p, 0, n * sizeof(float));
 		allocs.push_back(p);
+		if (hipMemset(p, 0, n * sizeof(float)) != hipSuccess) return nullptr;
 		return (float*)p;
 	}
 	bf16* alloc_b(size_t n)
 	{
 		void* p = nullptr;
 		if (hipMalloc(&p, n * sizeof(bf16)) != hipSuccess) return nullptr;
-		hipMemset(p, 0, n * sizeof(bf16));
 		allocs.push_back(p);
+		if (hipMemset(p, 0, n * sizeof(bf16)) != hipSuccess) return nullptr;
 		return (bf16*)p;
 	}
 	~qbf_cnn()
