@@ -11,10 +11,13 @@ transport, over all Nx=512 cells on one GPU: 1024 collision evals per step.  Pat
   fp64       FP64 plain: the reference's arithmetic.
 
 --warmup W steps run from the initial state and are discarded; then exactly --steps K steps of
-the fast path are timed from the initial state.  After K steps (200 by default) the density and
-energy per cell are compared with the FP64 plain trajectory in tests/golden/periodic64_fp64_ref.npz
-(frames every 20 steps).  Inputs are synthetic (qbf.workloads.periodic, seed 3), resident in HBM
-before the timed region.  `pcie_inclusive` adds the host-buffer boundary's per-step copies (state in
+the fast path are timed from the initial state.  The accuracy check is made after 200 steps of the
+same trajectory (continued untimed when K < 200): the density and energy per cell against the FP64
+plain trajectory in tests/golden/periodic64_fp64_ref.npz (frames every 20 steps), and the whole
+state on the four cells whose FP64 state that file keeps.  The two network-free fillers (collision
+dropped, last exact operator frozen) run the same 200 steps at the same cadence beside it, timed
+on their own.  Inputs are synthetic (qbf.workloads.periodic, seed 3), resident in HBM before the
+timed region.  `pcie_inclusive` adds the host-buffer boundary's per-step copies (state in
 and out, pinned memory, timed separately) to the fast path's step time; it is never `value`.
 
 The CPU baseline (the oracle, a C restatement of the reference algorithm) is timed first, before
@@ -209,10 +212,21 @@ def main():
                 "momentum_max_rel_thermal": float(np.abs(m[:, 4:6] - mr[:, 4:6]).max()
                                                   / np.sqrt(2 * np.abs(mr[:, 0] * mr[:, 6])).max()),
                 "density_l1_rel": float(np.abs(m[:, 0] - mr[:, 0]).sum() / np.abs(mr[:, 0]).sum()),
-                "energy_l1_rel": float(np.abs(m[:, 6] - mr[:, 6]).sum() / np.abs(mr[:, 6]).sum())}
+                "energy_l1_rel": float(np.abs(m[:, 6] - mr[:, 6]).sum() / np.abs(mr[:, 6]).sum()),
+                **state_error(W, step_no)}
+
+    def state_error(W, step_no):
+        """L1 error of the whole state on the cells whose FP64 state the reference keeps (last frame)."""
+        if ref is None or step_no != int(ref["steps"]):
+            return {}
+        Wk = W[torch.from_numpy(ref["cells"]).to(W.device)].cpu().numpy()
+        return {"state_l1_rel_kept_cells": float(np.abs(Wk - ref["final"]).sum() / np.abs(ref["final"]).sum()),
+                "kept_cells": [int(c) for c in ref["cells"]]}
 
     out = {}
-    check_step = args.steps if ref is not None and args.steps % int(ref["every"]) == 0 else None
+    # the accuracy check: the step of the reference's last frame (200), reached by the fast path's own
+    # trajectory (the timed steps, continued untimed when there are fewer)
+    check_step = int(ref["steps"]) if ref is not None else None
 
     # ---------------------------------------------------------------- FP32 exact path
     e32 = make(qbf.FP32)
@@ -290,15 +304,41 @@ def main():
         e32.profile(True)
         t0 = time.perf_counter()
         Ws, kept = run_steps(lambda a, b: st.step(a, out=b), W0, args.steps,
-                             keep_at=(check_step,) if check_step else ())
+                             keep_at=(check_step,) if check_step and check_step <= args.steps else ())
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
         kms, nl, kapps = e32.profile_read()
         e32.profile(False)
-        err = moment_errors(kept[check_step], check_step) if check_step else None
+        counts_timed = dict(st.counts)
+        err = None
+        if check_step:
+            if check_step > args.steps:
+                progress(f"continuing the trajectory to step {check_step} (untimed) for the accuracy check")
+                Wc, _ = run_steps(lambda a, b: st.step(a, out=b), Ws, check_step - args.steps)
+            else:
+                Wc = kept[check_step]
+            err = moment_errors(Wc, check_step)
+            err["policy_counts_to_check"] = dict(st.counts)
+            del Wc
+        # the network-free fillers at the same cadence (exact step every exact_every, conservation
+        # correction), each timed over its own trajectory to the check step
+        fillers = {}
+        for fl in ("frozen", "none") if check_step else ():
+            stf = S.SurrogateStepper(e32, None, isc, osc, wl.L_BOX, h, dt, exact_every=args.exact_every,
+                                     mask=qbf.CONS_ALL, filler=fl)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            Wf, _ = run_steps(lambda a, b: stf.step(a, out=b), W0, check_step)
+            torch.cuda.synchronize()
+            tf = time.perf_counter() - t0
+            fillers[fl] = {"value": round(evals_per_step * check_step / tf, 3), "steps": check_step,
+                           "counts": dict(stf.counts), "error_vs_fp64": moment_errors(Wf, check_step)}
+            progress(f"filler {fl}: {fillers[fl]['value']:.0f} evals/s")
+            del Wf
         out["surrogate"] = {
             "value": evals_per_step * args.steps / ts, "ms_per_step": 1e3 * ts / args.steps, "steps": args.steps,
-            "i3": (kms, nl, kapps, ts), "error_vs_fp64": err, "counts": dict(st.counts),
+            "i3": (kms, nl, kapps, ts), "error_vs_fp64": err, "counts": counts_timed,
+            "network_free_fillers": fillers,
             "residual_median": float(np.median(st.residuals)) if st.residuals else None,
             "residual_max": float(np.max(st.residuals)) if st.residuals else None,
             "operator_residual_median": float(np.median(st.op_residuals)) if st.op_residuals else None,
@@ -351,7 +391,7 @@ def main():
                                   "FP32 convolution kernels, FP64 state, midpoint update and conservation correction"),
                    "path": fast, "N": N, "Nx": Nx, "J": J, "M": M, "evals_per_step": evals_per_step,
                    "i3_pairs": pairs_info},
-        "roofline": {"bound": "mfma", "kernel": i3_kernel,
+        "roofline": {"bound": "valu", "kernel": i3_kernel,
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp32"], 4),
                      "traffic": (I3_TRAFFIC_BYTES[bool(used)] if N == 64 else

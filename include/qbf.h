@@ -121,7 +121,10 @@ int qbf_last_collision(qbf_engine *e, int ncell, double *dWin, double *dC);
  * reference.  The step runs once eagerly first (dOut holds its result), then is captured on the
  * engine's stream, which must not be the null stream (qbf_set_stream).  Needs a fixed paired-path
  * choice (qbf_set_real_pairs mode 0 or 2) and no buffer growth during the capture.  Launches replay
- * on that stream, asynchronously. */
+ * on the capture stream, asynchronously, ordered after the work queued on the engine's current
+ * stream and before the work queued there afterwards.  The graph uses the engine's buffers: while
+ * it is alive, calls that would grow a buffer or drop a cached table fail (-1); after
+ * qbf_engine_destroy the graph is dead (qbf_graph_launch fails, qbf_graph_destroy frees it). */
 typedef struct qbf_graph qbf_graph;
 int  qbf_graph_capture_inhom_step_periodic(qbf_engine *e, const double *dW, int ncell, double L, double h,
 	double dt, const double *dBext, double *dOut, qbf_graph **g);
@@ -232,10 +235,11 @@ int    qbf_surrogate_residual(qbf_cnn *c, qbf_engine *e, const double *dW, int n
 	int nsample, float iscale, float oscale, double sums[3]);
 /* Offset field of the surrogate operator: C(W) ~ oscale * net(iscale * W) + D in the calls above and
  * below (D: fp32 planar [ncell][4][N][N] device array on the same cells as their states, kept by
- * pointer until the next call; NULL removes it).  With D = C(W0) - oscale * net(iscale * W0) at an
- * exact step's state W0 (qbf_last_collision) the surrogate follows the exact operator there and
- * the network supplies only its change. */
-int    qbf_surrogate_set_offset(qbf_cnn *c, const float *dD);
+ * pointer until the next call; NULL removes it).  While D is set, the stepping and residual calls
+ * reject any other cell count.  With D = C(W0) - oscale * net(iscale * W0) at an exact step's
+ * state W0 (qbf_last_collision) the surrogate follows the exact operator there and the network
+ * supplies only its change. */
+int    qbf_surrogate_set_offset(qbf_cnn *c, const float *dD, int ncell);
 /* Strang step as qbf_inhom_step_periodic with the surrogate collision step; ncell <= maxBatch. */
 int    qbf_surrogate_inhom_step_periodic(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double L,
 	double h, double dt, float iscale, float oscale, int mask, double *dOut);

@@ -18,7 +18,9 @@
 // constant channel iscale * a_b, its output is scaled by oscale * a_b^2, and the training loss
 // is taken on the normalised output.
 #include <hip/hip_runtime.h>
+#include <stdarg.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <string>
@@ -429,15 +431,20 @@ __global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ 
 using namespace qbfcnn;
 
 static thread_local std::string g_cnn_err;
-static int cnn_err(int c, const char* m)
+static int cnn_err(int c, const char* fmt, ...)
 {
-	g_cnn_err = m;
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_cnn_err = buf;
 	return c;
 }
 #define HCHK(x)                                                              \
 	do {                                                                     \
 		hipError_t e_ = (x);                                                 \
-		if (e_ != hipSuccess) return cnn_err(-2, hipGetErrorString(e_));    \
+		if (e_ != hipSuccess) return cnn_err(-2, "%s", hipGetErrorString(e_));  \
 	} while (0)
 
 struct qbf_cnn {
@@ -469,6 +476,7 @@ struct qbf_cnn {
 	double *swf = nullptr, *swc = nullptr; // Fourier state / exact collision of the sample (complex)
 	// offset field D (fp32 [ncell][4][S][S], the caller's): the surrogate operator is net + D
 	const float* off = nullptr;
+	int offCells = 0;                     // the cell count D was given for
 	float* soff = nullptr;                // D on the residual sample
 	double* sres = nullptr;               // three sums of the residual
 	int step = 0;
@@ -760,6 +768,8 @@ int qbf_surrogate_collision_step(qbf_cnn* c, qbf_engine* e, const double* dW, in
 {
 	if (!c || !e || !dW || !dOut || ncell < 1) return cnn_err(-1, "qbf_surrogate_collision_step: bad argument");
 	if (qbf_engine_N(e) != c->S) return cnn_err(-1, "qbf_surrogate_collision_step: engine and network grids differ");
+	if (c->off && ncell != c->offCells)
+		return cnn_err(-1, "qbf_surrogate_collision_step: %d cells, but the offset field holds %d", ncell, c->offCells);
 	const int S = c->S;
 	if (dW == dOut) return cnn_err(-1, "qbf_surrogate_collision_step: in-place update not supported");
 	hipStream_t saved = c->stream;
@@ -781,7 +791,7 @@ int qbf_surrogate_collision_step(qbf_cnn* c, qbf_engine* e, const double* dW, in
 		if (hipGetLastError() != hipSuccess) r = cnn_err(-2, "surrogate step launch failed");
 	}
 	if (!r && mask && qbf_moment_correct(e, dW, dOut, ncell, L, mask))
-		r = cnn_err(-2, qbf_last_error());
+		r = cnn_err(-2, "%s", qbf_last_error());
 	c->stream = saved;
 	return r;
 }
@@ -798,6 +808,8 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 	if (!c || !e || !dW || !hostIdx || !sums || nsample < 1 || nsample > qbf_cnn::RMAX || nsample > c->maxB)
 		return cnn_err(-1, "qbf_surrogate_residual: bad argument (1 <= nsample <= 64 and <= maxBatch)");
 	if (qbf_engine_N(e) != c->S) return cnn_err(-1, "qbf_surrogate_residual: engine and network grids differ");
+	if (c->off && ncell != c->offCells)
+		return cnn_err(-1, "qbf_surrogate_residual: %d cells, but the offset field holds %d", ncell, c->offCells);
 	const int S = c->S;
 	for (int j = 0; j < nsample; j++)
 		if (hostIdx[j] < 0 || hostIdx[j] >= ncell) return cnn_err(-1, "qbf_surrogate_residual: cell index out of range");
@@ -817,7 +829,7 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 		if (qbf_phys_to_fourier(e, c->sw, CREAL * nsample, c->swf) ||
 			qbf_collision(e, c->swf, nsample, QBF_C, c->swc, nullptr, nullptr, nullptr, 1.0) ||
 			qbf_fourier_to_phys(e, c->swc, CREAL * nsample, c->sw2))
-			r = cnn_err(-2, qbf_last_error());
+			r = cnn_err(-2, "%s", qbf_last_error());
 	}
 	if (!r) {
 		hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->sw, c->sx, n);
@@ -838,11 +850,14 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 
 // Offset field of the surrogate operator: C(W) ~ oscale * net(iscale * W) + D (D fp32, planar
 // [ncell][4][N][N] device array, the same cells as the states passed to the surrogate calls; the
-// pointer is kept until the next call; NULL: no offset).
-int qbf_surrogate_set_offset(qbf_cnn* c, const float* dD)
+// pointer is kept until the next call; NULL: no offset).  The stepping and residual calls reject
+// another cell count while D is set.
+int qbf_surrogate_set_offset(qbf_cnn* c, const float* dD, int ncell)
 {
 	if (!c) return cnn_err(-1, "qbf_surrogate_set_offset: null network");
+	if (dD && ncell < 1) return cnn_err(-1, "qbf_surrogate_set_offset: ncell must be >= 1");
 	c->off = dD;
+	c->offCells = dD ? ncell : 0;
 	return 0;
 }
 
@@ -854,10 +869,10 @@ int qbf_surrogate_inhom_step_periodic(qbf_cnn* c, qbf_engine* e, const double* d
 {
 	if (!c || !e || !dW || !dOut || ncell < 1 || ncell > c->maxB)
 		return cnn_err(-1, "qbf_surrogate_inhom_step_periodic: bad argument (ncell must be in [1, maxBatch])");
-	if (qbf_transport_periodic(e, dW, ncell, L, h, 0.5 * dt, c->sw)) return cnn_err(-2, qbf_last_error());
+	if (qbf_transport_periodic(e, dW, ncell, L, h, 0.5 * dt, c->sw)) return cnn_err(-2, "%s", qbf_last_error());
 	int r = qbf_surrogate_collision_step(c, e, c->sw, ncell, dt, iscale, oscale, L, mask, c->sw2);
 	if (r) return r;
-	if (qbf_transport_periodic(e, c->sw2, ncell, L, h, 0.5 * dt, dOut)) return cnn_err(-2, qbf_last_error());
+	if (qbf_transport_periodic(e, c->sw2, ncell, L, h, 0.5 * dt, dOut)) return cnn_err(-2, "%s", qbf_last_error());
 	return 0;
 }
 

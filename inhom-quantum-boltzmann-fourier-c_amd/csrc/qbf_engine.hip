@@ -76,6 +76,24 @@ template <> struct CfgNN<64> { static constexpr int T = 256; static constexpr bo
 // an FP64 128 x 128 state grid (256 KiB) exchanges through global memory as well
 template <> struct CfgNN<128> { static constexpr int T = 1024; static constexpr bool XG = true; };
 
+struct qbf_graph {
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipStream_t stream = nullptr;    // the engine's stream at capture time; replays launch there
+	struct qbf_engine* e = nullptr;         // null once the engine is destroyed (the graph is then dead)
+	hipEvent_t before = nullptr, after = nullptr;   // order replays against the engine's current stream
+	void release()
+	{
+		if (exec) (void)hipGraphExecDestroy(exec);
+		if (graph) (void)hipGraphDestroy(graph);
+		if (before) (void)hipEventDestroy(before);
+		if (after) (void)hipEventDestroy(after);
+		exec = nullptr;
+		graph = nullptr;
+		before = after = nullptr;
+	}
+};
+
 struct qbf_engine {
 	virtual ~qbf_engine()
 	{
@@ -112,6 +130,12 @@ struct qbf_engine {
 	}
 	hipStream_t stream = nullptr;
 	bool ownStream = false;
+	// HIP graphs captured from this engine hold raw pointers to its buffers and cached tables: while
+	// any is alive (liveGraphs > 0) no buffer may grow (be reallocated) and no cached table may be
+	// dropped; the engine's destruction invalidates the graphs still alive (graphs)
+	int liveGraphs = 0;
+	std::vector<qbf_graph*> graphs;
+	std::vector<hipStream_t> retired;   // own streams replaced while a graph captured on them was alive
 	virtual int set_i1(int slot, unsigned num, const double* p1, const double* p2) = 0;
 	virtual int set_i2(unsigned num, const double* p1, const double* p2) = 0;
 	virtual int set_i3(unsigned num, const double* p1, const double* p2) = 0;
@@ -174,10 +198,14 @@ namespace {
 struct DevBuf {
 	void* p = nullptr;
 	size_t bytes = 0;
+	const int* lock = nullptr;     // the owner's live-graph count: no reallocation while it is non-zero
 	int ensure(size_t n)
 	{
 		if (n <= bytes) return 0;
 		if (tl_capturing) return set_err(-1, "buffer growth during graph capture");
+		if (lock && *lock)
+			return set_err(-1, "buffer growth (%zu bytes) while a captured graph uses the engine's buffers: "
+				"destroy the graph first, or capture after a call of the largest size", n);
 		if (p) hipFree(p);
 		p = nullptr;
 		bytes = 0;
@@ -238,9 +266,11 @@ struct EngineT final : public qbf_engine {
 
 	int init()
 	{
-		int r = ensure_twiddles();
-		if (r) return r;
-		return r;
+		for (DevBuf* b : {&i1[0].phi, &i1[0].psi1n, &i1[1].phi, &i1[1].psi1n, &i2p1, &i2p2, &i2pc, &i3p1, &i3p2n,
+			     &What, &Ahat, &Bhat, &Sbuf, &acc3, &Fscr, &xscr, &Wmid, &WF, &WF2, &Wt1, &Wt2, &src3, &Wpk,
+			     &WhatP, &reim, &Ipk, &xscrN, &Cf})
+			b->lock = &liveGraphs;
+		return ensure_twiddles();
 	}
 
 	// --------------------------------------------------------------- tables
@@ -619,6 +649,8 @@ struct EngineT final : public qbf_engine {
 				return 0;
 			}
 		if (tl_capturing) return set_err(-1, "combination table not prepared: run the step once before capturing it");
+		if (termCache.size() >= TERM_CACHE && liveGraphs)
+			return set_err(-1, "combination-table cache full while a captured graph uses its tables: destroy the graph first");
 		if (termCache.size() >= TERM_CACHE) {
 			HIPCHK(hipStreamSynchronize(stream));     // no kernel reads the table that is dropped
 			termCache.pop_front();
@@ -1060,6 +1092,15 @@ void qbf_engine_destroy(qbf_engine* e)
 {
 	if (!e) return;
 	if (e->stream) hipStreamSynchronize(e->stream);
+	// graphs still alive lose their executable graph (their buffers go with the engine); the handles
+	// stay valid until qbf_graph_destroy, and qbf_graph_launch on them returns an error
+	for (qbf_graph* g : e->graphs) {
+		if (g->stream) (void)hipStreamSynchronize(g->stream);
+		g->release();
+		g->e = nullptr;
+	}
+	e->graphs.clear();
+	for (hipStream_t s : e->retired) hipStreamDestroy(s);
 	if (e->ownStream && e->stream) hipStreamDestroy(e->stream);
 	delete e;
 }
@@ -1074,7 +1115,8 @@ int qbf_set_stream(qbf_engine* e, void* s)
 	if (!e) return set_err(-1, "null engine");
 	if (e->ownStream && e->stream) {
 		hipStreamSynchronize(e->stream);
-		hipStreamDestroy(e->stream);
+		if (e->liveGraphs) e->retired.push_back(e->stream);   // a live graph replays on it
+		else hipStreamDestroy(e->stream);
 	}
 	if (s) {
 		e->stream = (hipStream_t)s;
@@ -1206,11 +1248,6 @@ int qbf_last_collision(qbf_engine* e, int ncell, double* dWin, double* dC)
 	return e->last_collision(ncell, dWin, dC);
 }
 
-struct qbf_graph {
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t exec = nullptr;
-	hipStream_t stream = nullptr;
-};
 
 // One Strang step (qbf_inhom_step_periodic with these buffers) as a HIP graph: the step runs once
 // eagerly (every buffer sized, every combination table uploaded), then is captured on the
@@ -1240,29 +1277,62 @@ int qbf_graph_capture_inhom_step_periodic(qbf_engine* e, const double* W, int nc
 		if (graph) hipGraphDestroy(graph);
 		return r ? r : set_err(-2, "hipStreamEndCapture failed: %s", hipGetErrorString(ce));
 	}
-	hipGraphExec_t exec = nullptr;
-	const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+	auto* ng = new qbf_graph();
+	ng->graph = graph;
+	ng->stream = e->stream;
+	ng->e = e;
+	const hipError_t ie = hipGraphInstantiate(&ng->exec, graph, nullptr, nullptr, 0);
 	if (ie != hipSuccess) {
-		hipGraphDestroy(graph);
+		ng->release();
+		delete ng;
 		return set_err(-2, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
 	}
-	*g = new qbf_graph{graph, exec, e->stream};
+	if (hipEventCreateWithFlags(&ng->before, hipEventDisableTiming) != hipSuccess ||
+		hipEventCreateWithFlags(&ng->after, hipEventDisableTiming) != hipSuccess) {
+		ng->release();
+		delete ng;
+		return set_err(-2, "hipEventCreate failed");
+	}
+	e->liveGraphs++;
+	e->graphs.push_back(ng);
+	*g = ng;
 	return 0;
 }
 
+// A replay runs on the capture stream, after the work already queued on the engine's current stream
+// and before any work queued there later (the graph shares the engine's buffers with eager calls).
 int qbf_graph_launch(qbf_graph* g)
 {
 	if (!g) return set_err(-1, "null graph");
+	if (!g->e || !g->exec) return set_err(-1, "qbf_graph_launch: the engine of this graph was destroyed");
+	hipStream_t cur = g->e->stream;
+	const bool other = cur != g->stream;
+	if (other) {
+		HIPCHK(hipEventRecord(g->before, cur));
+		HIPCHK(hipStreamWaitEvent(g->stream, g->before, 0));
+	}
 	HIPCHK(hipGraphLaunch(g->exec, g->stream));
+	if (other) {
+		HIPCHK(hipEventRecord(g->after, g->stream));
+		HIPCHK(hipStreamWaitEvent(cur, g->after, 0));
+	}
 	return 0;
 }
 
 void qbf_graph_destroy(qbf_graph* g)
 {
 	if (!g) return;
-	hipStreamSynchronize(g->stream);
-	hipGraphExecDestroy(g->exec);
-	hipGraphDestroy(g->graph);
+	if (g->e) {
+		if (g->stream) (void)hipStreamSynchronize(g->stream);
+		auto& v = g->e->graphs;
+		for (size_t i = 0; i < v.size(); i++)
+			if (v[i] == g) {
+				v.erase(v.begin() + (long)i);
+				break;
+			}
+		g->e->liveGraphs--;
+	}
+	g->release();
 	delete g;
 }
 

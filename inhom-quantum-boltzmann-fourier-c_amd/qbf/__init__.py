@@ -111,12 +111,17 @@ def lib():
 
 
 class Graph:
-    """A captured step (Engine.capture_inhom_step_periodic); keeps its buffers and stream alive."""
+    """A captured step (Engine.capture_inhom_step_periodic); keeps its engine, buffers and stream
+    alive.  A replay runs after the work queued on the engine's current stream and before the work
+    queued there afterwards."""
 
-    def __init__(self, h, stream, keep):
-        self._h, self.stream, self._keep = h, stream, keep
+    def __init__(self, h, stream, keep, engine):
+        self._h, self.stream, self._keep, self._engine = h, stream, keep, engine
 
     def launch(self):
+        if not getattr(self, "_h", None):
+            raise QBFError("graph destroyed")
+        self._engine._bind()
         _check(lib().qbf_graph_launch(self._h))
 
     def close(self):
@@ -136,10 +141,17 @@ def _check(rc: int):
         raise QBFError(lib().qbf_last_error().decode())
 
 
-def _ptr(t):
+def _ptr(t, dtype=None, shape=None, what="tensor"):
+    """Device address of a contiguous cuda tensor (None passes through), after checking its dtype
+    (a torch dtype) and shape (a tuple; None entries are free) when given."""
     if t is None:
         return None
-    assert t.is_cuda and t.is_contiguous()
+    if not (t.is_cuda and t.is_contiguous()):
+        raise QBFError(f"{what}: expected a contiguous cuda tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise QBFError(f"{what}: expected {dtype}, got {t.dtype}")
+    if shape is not None and (t.dim() != len(shape) or any(a is not None and a != b for a, b in zip(shape, t.shape))):
+        raise QBFError(f"{what}: expected shape {shape}, got {tuple(t.shape)}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -211,7 +223,8 @@ class Engine:
 
     def moment_correct(self, Wold, Wnew, L, mask=CONS_ALL):
         self._bind()
-        _check(lib().qbf_moment_correct(self._h, _ptr(Wold), _ptr(Wnew), Wold.shape[0], L, mask))
+        n = Wold.shape[0]
+        _check(lib().qbf_moment_correct(self._h, self._R(Wold, "Wold"), self._R(Wnew, "Wnew", n), n, L, mask))
         return Wnew
 
     def simulate_periodic(self, W0, L, h, dt, numsteps, stride=1, Bext=None):
@@ -220,7 +233,8 @@ class Engine:
         self._bind()
         nf = (numsteps - 1) // stride + 1 if numsteps > 0 else 0
         out = np.empty((max(nf, 1),) + tuple(W0.shape))
-        _check(lib().qbf_simulate_periodic(self._h, _ptr(W0), W0.shape[0], L, h, dt, numsteps, stride, _ptr(Bext),
+        _check(lib().qbf_simulate_periodic(self._h, self._R(W0), W0.shape[0], L, h, dt, numsteps, stride,
+                                           self._B(Bext, W0.shape[0]),
                                            out.ctypes.data_as(C.c_void_p)))
         return out[:nf]
 
@@ -229,7 +243,7 @@ class Engine:
         min and max eigenvalue per cell of physical states W."""
         self._bind()
         out = self._torch.empty((W.shape[0], 10), dtype=self._torch.float64, device=W.device)
-        _check(lib().qbf_cell_diagnostics(self._h, _ptr(W), W.shape[0], L, _ptr(out)))
+        _check(lib().qbf_cell_diagnostics(self._h, self._R(W), W.shape[0], L, _ptr(out)))
         return out
 
     def set_real_pairs(self, mode: int, tol: float = 0.0):
@@ -250,7 +264,7 @@ class Engine:
         t = self._torch
         prm = t.empty((W.shape[0], 12), dtype=t.float64, device=W.device)
         wfd = t.empty_like(W) if with_states else None
-        _check(lib().qbf_fermi_dirac(self._h, _ptr(W), W.shape[0], L, _ptr(prm), _ptr(wfd)))
+        _check(lib().qbf_fermi_dirac(self._h, self._R(W), W.shape[0], L, _ptr(prm), _ptr(wfd)))
         return prm, wfd
 
     def fd_distance(self, W, L, WFD):
@@ -260,20 +274,48 @@ class Engine:
         t = self._torch
         stride = 1 if (WFD.dim() == 4 and WFD.shape[0] == W.shape[0] and W.shape[0] > 1) else 0
         out = t.empty((W.shape[0], 2), dtype=t.float64, device=W.device)
-        _check(lib().qbf_fd_distance(self._h, _ptr(W), W.shape[0], L, _ptr(WFD.contiguous()), stride, _ptr(out)))
+        WFD = WFD.contiguous()
+        if WFD.dtype != t.float64 or WFD.numel() != (W.shape[0] if stride else 1) * 4 * self.N * self.N:
+            raise QBFError("WFD: expected float64 [4, N, N] or one state per cell")
+        _check(lib().qbf_fd_distance(self._h, self._R(W), W.shape[0], L, _ptr(WFD), stride, _ptr(out)))
         return out
 
     def cell_average(self, W):
         """[1, 4, N, N]: the average of the states of W over the cells."""
         self._bind()
         out = self._torch.empty((1,) + tuple(W.shape[1:]), dtype=W.dtype, device=W.device)
-        _check(lib().qbf_cell_average(self._h, _ptr(W), W.shape[0], _ptr(out)))
+        _check(lib().qbf_cell_average(self._h, self._R(W), W.shape[0], _ptr(out)))
         return out
 
     # --------------------------------------------------------------- stream
     def _bind(self):
+        if not getattr(self, "_h", None):
+            raise QBFError("engine closed")
         s = self._torch.cuda.current_stream().cuda_stream
         _check(lib().qbf_set_stream(self._h, C.c_void_p(s)))
+
+    # checked device addresses: states [ncell, 4, N, N] (Fourier: complex128, physical: float64),
+    # single grids [N, N] complex128, per-cell magnetic fields [ncell, 3] float64
+    def _F(self, t, what="Fourier state", n=None):
+        return _ptr(t, self._torch.complex128, (n, 4, self.N, self.N), what)
+
+    def _R(self, t, what="physical state", n=None):
+        return _ptr(t, self._torch.float64, (n, 4, self.N, self.N), what)
+
+    def _G(self, t, what="grid"):
+        if t is not None and t.numel() != self.N * self.N:
+            raise QBFError(f"{what}: expected {self.N} x {self.N} elements, got {t.numel()}")
+        return _ptr(t, self._torch.complex128, None, what)
+
+    def _R1(self, t, what):
+        if t is None or t.numel() != 4 * self.N * self.N:
+            raise QBFError(f"{what}: expected one physical state [4, {self.N}, {self.N}]")
+        return _ptr(t, self._torch.float64, None, what)
+
+    def _B(self, t, n):
+        if t is not None and t.numel() != 3 * n:
+            raise QBFError(f"Bext: expected {3 * n} values ([ncell, 3]), got {t.numel()}")
+        return _ptr(t, self._torch.float64, None, "Bext")
 
     # --------------------------------------------------------------- tables (numpy, reference layout)
     def set_quadrature(self, I1, I2, I3, I4):
@@ -307,84 +349,95 @@ class Engine:
         torch = self._torch
         self._bind()
         out = torch.empty_like(W) if out is None else out
-        _check(lib().qbf_collision(self._h, _ptr(W), W.shape[0], mode, _ptr(out), _ptr(base), _ptr(Wcb),
-                                   _ptr(Bext), cf))
+        n = W.shape[0]
+        _check(lib().qbf_collision(self._h, self._F(W), n, mode, self._F(out, "out", n), self._F(base, "base", n),
+                                   self._F(Wcb, "Wcb", n), self._B(Bext, n), cf))
         return out
 
     def collision_step(self, W, dt, Bext=None, out=None):
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_collision_step(self._h, _ptr(W), W.shape[0], dt, _ptr(Bext), _ptr(out)))
+        n = W.shape[0]
+        _check(lib().qbf_collision_step(self._h, self._F(W), n, dt, self._B(Bext, n), self._F(out, "out", n)))
         return out
 
     def I1(self, f, g, h, slot=TAB_I1):
         self._bind()
         out = self._torch.empty_like(f)
-        _check(lib().qbf_I1(self._h, slot, _ptr(f), _ptr(g), _ptr(h), _ptr(out)))
+        _check(lib().qbf_I1(self._h, slot, self._G(f, "f"), self._G(g, "g"), self._G(h, "h"), _ptr(out)))
         return out
 
     def I2(self, f, g, h):
         self._bind()
         out = self._torch.empty_like(f)
-        _check(lib().qbf_I2(self._h, _ptr(f), _ptr(g), _ptr(h), _ptr(out)))
+        _check(lib().qbf_I2(self._h, self._G(f, "f"), self._G(g, "g"), self._G(h, "h"), _ptr(out)))
         return out
 
     def I3(self, f, g, h):
         self._bind()
         out = self._torch.empty_like(f)
-        _check(lib().qbf_I3(self._h, _ptr(f), _ptr(g), _ptr(h), _ptr(out)))
+        _check(lib().qbf_I3(self._h, self._G(f, "f"), self._G(g, "g"), self._G(h, "h"), _ptr(out)))
         return out
 
     def phys_to_fourier(self, Wp):
         self._bind()
         out = self._torch.empty(Wp.shape, dtype=self._torch.complex128, device=Wp.device)
         ng = Wp.numel() // (self.N * self.N)
-        _check(lib().qbf_phys_to_fourier(self._h, _ptr(Wp), ng, _ptr(out)))
+        if ng * self.N * self.N != Wp.numel() or ng < 1:
+            raise QBFError(f"phys_to_fourier: {Wp.numel()} elements are not whole {self.N} x {self.N} grids")
+        _check(lib().qbf_phys_to_fourier(self._h, _ptr(Wp, self._torch.float64, None, "physical grids"), ng, _ptr(out)))
         return out
 
     def fourier_to_phys(self, WF):
         self._bind()
         out = self._torch.empty(WF.shape, dtype=self._torch.float64, device=WF.device)
         ng = WF.numel() // (self.N * self.N)
-        _check(lib().qbf_fourier_to_phys(self._h, _ptr(WF), ng, _ptr(out)))
+        if ng * self.N * self.N != WF.numel() or ng < 1:
+            raise QBFError(f"fourier_to_phys: {WF.numel()} elements are not whole {self.N} x {self.N} grids")
+        _check(lib().qbf_fourier_to_phys(self._h, _ptr(WF, self._torch.complex128, None, "Fourier grids"), ng, _ptr(out)))
         return out
 
     def transport_periodic(self, W, L, h, dt, out=None):
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_transport_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(out)))
+        _check(lib().qbf_transport_periodic(self._h, self._R(W), W.shape[0], L, h, dt, self._R(out, "out", W.shape[0])))
         return out
 
     def transport_dirichlet(self, W, L, h, dt, out=None):
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_transport_dirichlet(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(out)))
+        _check(lib().qbf_transport_dirichlet(self._h, self._R(W), W.shape[0], L, h, dt, self._R(out, "out", W.shape[0])))
         return out
 
     def inhom_step_dirichlet(self, W, L, h, dt, Bext=None, out=None):
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_inhom_step_dirichlet(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
+        n = W.shape[0]
+        _check(lib().qbf_inhom_step_dirichlet(self._h, self._R(W), n, L, h, dt, self._B(Bext, n), self._R(out, "out", n)))
         return out
 
     def transport_maxwell(self, W, L, h, dt, lam, WL, WR, out=None):
         """Slope-limited transport with Maxwell reflection; WL, WR: boundary states [4, N, N] on the device."""
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_transport_maxwell(self._h, _ptr(W), W.shape[0], L, h, dt, lam, _ptr(WL), _ptr(WR), _ptr(out)))
+        n = W.shape[0]
+        _check(lib().qbf_transport_maxwell(self._h, self._R(W), n, L, h, dt, lam, self._R1(WL, "WL"), self._R1(WR, "WR"),
+                                           self._R(out, "out", n)))
         return out
 
     def inhom_step_maxwell(self, W, L, h, dt, lam, WL, WR, Bext=None, out=None):
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_inhom_step_maxwell(self._h, _ptr(W), W.shape[0], L, h, dt, lam, _ptr(WL), _ptr(WR),
-                                            _ptr(Bext), _ptr(out)))
+        n = W.shape[0]
+        _check(lib().qbf_inhom_step_maxwell(self._h, self._R(W), n, L, h, dt, lam, self._R1(WL, "WL"), self._R1(WR, "WR"),
+                                            self._B(Bext, n), self._R(out, "out", n)))
         return out
 
     def inhom_step_periodic(self, W, L, h, dt, Bext=None, out=None):
         self._bind()
         out = self._torch.empty_like(W) if out is None else out
-        _check(lib().qbf_inhom_step_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext), _ptr(out)))
+        n = W.shape[0]
+        _check(lib().qbf_inhom_step_periodic(self._h, self._R(W), n, L, h, dt, self._B(Bext, n), self._R(out, "out", n)))
         return out
 
     def last_collision(self, ncell, with_state=True):
@@ -409,6 +462,7 @@ class Engine:
         g = C.c_void_p()
         with torch.cuda.stream(stream):
             self._bind()
-            _check(lib().qbf_graph_capture_inhom_step_periodic(self._h, _ptr(W), W.shape[0], L, h, dt, _ptr(Bext),
-                                                               _ptr(out), C.byref(g)))
-        return Graph(g, stream, (W, out, Bext)), out
+            n = W.shape[0]
+            _check(lib().qbf_graph_capture_inhom_step_periodic(self._h, self._R(W), n, L, h, dt, self._B(Bext, n),
+                                                               self._R(out, "out", n), C.byref(g)))
+        return Graph(g, stream, (W, out, Bext), self), out

@@ -40,7 +40,7 @@ def _lib():
             "qbf_surrogate_residual": (i, [vp, vp, vp, i, C.POINTER(i), i, f, f, C.POINTER(C.c_double)]),
             "qbf_surrogate_inhom_step_periodic": (i, [vp, vp, vp, i, C.c_double, C.c_double, C.c_double, f, f, i,
                                                       vp]),
-            "qbf_surrogate_set_offset": (i, [vp, vp]),
+            "qbf_surrogate_set_offset": (i, [vp, vp, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -56,7 +56,8 @@ def _check(rc):
 
 
 def _ptr(t):
-    assert t.is_cuda and t.is_contiguous() and t.dtype.is_floating_point
+    if not (t.is_cuda and t.is_contiguous() and t.dtype == __import__("torch").float32):
+        raise QBFError(f"expected a contiguous float32 cuda tensor, got {t.dtype} on {t.device}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -155,8 +156,10 @@ class CNN:
     def set_offset(self, D=None):
         """Offset field of the surrogate operator, C(W) ~ net(W) + D (float32 cuda [ncell, 4, N, N] on
         the cells of the states passed to the stepping calls), or None."""
+        if D is not None and (D.dtype != self._torch.float32 or D.dim() != 4 or tuple(D.shape[1:]) != (4, self.N, self.N)):
+            raise QBFError(f"offset field must be float32 [ncell, 4, {self.N}, {self.N}], got {D.dtype} {tuple(D.shape)}")
         self._offset = D
-        _check(_lib().qbf_surrogate_set_offset(self._h, None if D is None else _ptr(D)))
+        _check(_lib().qbf_surrogate_set_offset(self._h, None if D is None else _ptr(D), 0 if D is None else D.shape[0]))
 
     def collision_step(self, engine, W, dt, iscale, oscale, L, mask=0, out=None):
         """Midpoint collision step of physical states W (float64 cuda [ncell, 4, N, N]) with the
@@ -187,5 +190,6 @@ class CNN:
 
 
 def _ptrd(t):
-    assert t.is_cuda and t.is_contiguous() and t.dtype == __import__("torch").float64
+    if not (t.is_cuda and t.is_contiguous() and t.dtype == __import__("torch").float64):
+        raise QBFError(f"expected a contiguous float64 cuda tensor, got {t.dtype} on {t.device}")
     return C.c_void_p(t.data_ptr())

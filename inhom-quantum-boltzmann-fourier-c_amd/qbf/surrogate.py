@@ -21,6 +21,11 @@ Stepping policy (SurrogateStepper.step), per Strang step of the periodic problem
     with W0 the state the exact step's collision started from and C(W0) taken from that step
     (Engine.last_collision: no extra evaluation); the network then supplies only the change of
     the operator since the exact step, and its systematic error cancels.
+
+Network-free fillers (`filler`), the baselines that show what the network gains: between the
+exact steps the collision is either dropped ("none": transport only) or frozen at the last exact
+operator ("frozen": C(W) ~ C(W0), with the same conservation correction); neither runs a residual
+check.
 """
 from __future__ import annotations
 
@@ -127,14 +132,23 @@ def train(net, X, Y, iscale, oscale, steps=3000, batch=32, lr=2e-3, lr_min=2e-5,
 class SurrogateStepper:
     """Strang steps of the periodic problem with the surrogate collision step and its safeguards."""
 
+    FILLERS = ("network", "frozen", "none")
+
     def __init__(self, engine, net, iscale, oscale, L, h, dt, exact_every=10, sample_frac=0.02, tol=1e-3,
-                 mask=CONS_ALL, check=True, seed=0, anchor=True):
+                 mask=CONS_ALL, check=True, seed=0, anchor=True, filler="network"):
+        if filler not in self.FILLERS:
+            raise QBFError(f"filler must be one of {self.FILLERS}")
+        if filler != "network" and not exact_every:
+            raise QBFError("a network-free filler needs exact steps (exact_every >= 1)")
         self.e, self.net = engine, net
         self.iscale, self.oscale = iscale, oscale
         self.L, self.h, self.dt = L, h, dt
         self.exact_every, self.sample_frac, self.tol = exact_every, sample_frac, tol
-        self.mask, self.check, self.anchor = mask, check, anchor
-        net.set_offset(None)
+        self.filler = filler
+        self.mask, self.check, self.anchor = mask, check and filler == "network", anchor and filler == "network"
+        self.C0 = None               # the last exact operator (frozen filler)
+        if net is not None:
+            net.set_offset(None)
         self.rng = np.random.default_rng(seed)
         self.nstep = 0
         self.counts = {"exact": 0, "surrogate": 0, "fallback": 0}
@@ -144,6 +158,8 @@ class SurrogateStepper:
     def _exact(self, W, out):
         self.e.set_moment_correction(self.mask)
         out = self.e.inhom_step_periodic(W, self.L, self.h, self.dt, out=out)
+        if self.filler == "frozen":
+            self.C0 = self.e.last_collision(W.shape[0], with_state=False)[1]
         if self.anchor:
             W0, C0 = self.e.last_collision(W.shape[0])
             N0 = self.net.forward(W0.float().contiguous(), self.iscale, self.oscale)
@@ -159,14 +175,25 @@ class SurrogateStepper:
             ns = min(64, max(1, int(round(self.sample_frac * ncell))))
             cells = np.sort(self.rng.choice(ncell, ns, replace=False))
             d, c, w = self.net.residual(self.e, W, cells, self.iscale, self.oscale)
-            r = max(1, self.exact_every) * self.dt * d / w
+            # an empty or non-finite sample (w == 0, NaN or Inf anywhere in the sums) cannot vouch for
+            # the network: r is then infinite or NaN, and only a finite r <= tol skips the exact step
+            r = max(1, self.exact_every) * self.dt * d / w if w > 0 else math.inf
             self.residuals.append(r)
-            self.op_residuals.append(d / c if c > 0 else float("inf"))
-            if r > self.tol:
+            self.op_residuals.append(d / c if c > 0 else math.inf)
+            if not (math.isfinite(r) and r <= self.tol):
                 kind = "fallback"
-        if kind == "surrogate":
+        if kind == "surrogate" and self.filler == "network":
             out = self.net.inhom_step_periodic(self.e, W, self.L, self.h, self.dt, self.iscale, self.oscale,
                                                self.mask, out=out)
+        elif kind == "surrogate":
+            B = self.e.transport_periodic(W, self.L, self.h, 0.5 * self.dt)
+            if self.filler == "frozen":
+                # midpoint step with a constant operator: W + dt C0, then the conservation correction
+                Bn = B + self.dt * self.C0
+                if self.mask:
+                    self.e.moment_correct(B, Bn, self.L, self.mask)
+                B = Bn
+            out = self.e.transport_periodic(B, self.L, self.h, 0.5 * self.dt, out=out)
         else:
             out = self._exact(W, out)
         self.counts[kind] += 1

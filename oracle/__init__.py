@@ -57,7 +57,9 @@ def lib():
             "orc_r2c_fill": (None, [i, _dp, _dp]),
             "orc_c2r_real": (None, [i, _dp, _dp]),
             "orc_InhomStepPeriodic": (None, [C.c_void_p, u, d, _dp, d, d, _dp, _dp]),
+            "orc_InhomStepPeriodicCells": (None, [C.c_void_p, u, d, _dp, d, d, _dp, u, C.POINTER(u), _dp]),
             "orc_InhomStepDirichlet": (None, [C.c_void_p, u, d, _dp, d, d, _dp, _dp]),
+            "orc_TransportMaxwell": (None, [i, u, d, _dp, d, d, d, _dp, _dp, _dp]),
             "orc_InhomStepMaxwell": (None, [C.c_void_p, u, d, _dp, d, d, d, _dp, _dp, _dp, _dp]),
             "orc_SimulationHomogeneous": (None, [C.c_void_p, _dp, d, u, _dp, _dp]),
             "orc_SlopeLimiterStepPeriodic": (None, [d, d, d, u, _dp, _dp]),
@@ -261,6 +263,19 @@ def inhom_step_periodic(q: Quad, W, h, dt, Bext=None):
     return out
 
 
+def inhom_step_periodic_cells(q: Quad, W, h, dt, cells, Bext=None):
+    """inhom_step_periodic, exact on the listed cells only (collisions evaluated within two cells of
+    them); the other cells of the result are undefined."""
+    W = _c(W, np.float64)
+    nv = W.shape[0]
+    B = np.zeros(3 * nv) if Bext is None else _c(Bext, np.float64)
+    idx = np.ascontiguousarray(cells, np.uint32)
+    out = np.zeros_like(W)
+    lib().orc_InhomStepPeriodicCells(q.ptr, nv, q.L, _p(W), h, dt, _p(B), idx.size,
+                                     idx.ctypes.data_as(C.POINTER(C.c_uint)), _p(out))
+    return out
+
+
 def inhom_step_dirichlet(q: Quad, W, h, dt, Bext=None):
     W = _c(W, np.float64)
     nv = W.shape[0]
@@ -277,6 +292,15 @@ def inhom_step_maxwell(q: Quad, W, h, dt, lam, WL, WR, Bext=None):
     out = np.empty_like(W)
     lib().orc_InhomStepMaxwell(q.ptr, nv, q.L, _p(W), h, dt, lam, _p(_c(WL, np.float64)), _p(_c(WR, np.float64)),
                                _p(B), _p(out))
+    return out
+
+
+def transport_maxwell(W, L, h, dt, lam, WL, WR):
+    """One half transport (dt / 2) with Maxwell boundaries, as inhom_step_maxwell applies it."""
+    W = _c(W, np.float64)
+    out = np.empty_like(W)
+    lib().orc_TransportMaxwell(W.shape[-1], W.shape[0], L, _p(W), h, dt, lam, _p(_c(WL, np.float64)),
+                               _p(_c(WR, np.float64)), _p(out))
     return out
 
 

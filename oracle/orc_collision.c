@@ -358,6 +358,27 @@ void orc_InhomStepPeriodic(const orc_quad_t *q, unsigned numVol, double L, const
 	free(Wtmp2); free(Wtmp);
 }
 
+/* orc_InhomStepPeriodic evaluated on a subset: the collision runs only on the cells within two of
+ * a listed cell (the limiter's stencil, src/finite_volume.c:74-100, reaches two cells to each side),
+ * so Wnext is exact (bit for bit the full step's) on the listed cells and undefined elsewhere. */
+void orc_InhomStepPeriodicCells(const orc_quad_t *q, unsigned numVol, double L, const double *W,
+	double h, double dt, const double *Bext, unsigned nlist, const unsigned *list, double *Wnext)
+{
+	const int N = q->N;
+	const size_t cell = 4 * (size_t)N * N;
+	double *Wtmp = malloc(sizeof(double) * cell * numVol), *Wtmp2 = malloc(sizeof(double) * cell * numVol);
+	char *need = calloc(numVol, 1);
+	for (unsigned j = 0; j < nlist; j++)
+		for (int d = -2; d <= 2; d++) need[((int)list[j] + d + 2 * (int)numVol) % (int)numVol] = 1;
+	half_transport(N, numVol, L, h, dt, W, Wtmp);
+	memcpy(Wtmp2, Wtmp, sizeof(double) * cell * numVol);
+	#pragma omp parallel for schedule(dynamic, 1)
+	for (int i = 0; i < (int)numVol; i++)
+		if (need[i]) fourier_collision_cell(q, &Wtmp[i * cell], dt, &Bext[3 * i], &Wtmp2[i * cell]);
+	half_transport(N, numVol, L, h, dt, Wtmp2, Wnext);
+	free(need); free(Wtmp2); free(Wtmp);
+}
+
 static void half_transport_dirichlet(int N, unsigned numVol, double L, double h, double dt, const double *Win,
 	double *Wout)
 {
@@ -433,18 +454,14 @@ static void half_transport_maxwell(int N, unsigned numVol, const double *vxM, do
 	free(Wx1); free(Wx); free(tr);
 }
 
-/* Maxwell boundary step (src/simulation.c:882-1165): boundary states on the Maxwell grid,
- * normalised by their incoming trace flux; half transport, collision of every cell, half transport */
-void orc_InhomStepMaxwell(const orc_quad_t *q, unsigned numVol, double L, const double *W, double h, double dt,
-	double lambda, const double *WMaxwL, const double *WMaxwR, const double *Bext, double *Wnext)
+/* Maxwell boundary states on the Maxwell grid, normalised by their incoming trace flux
+ * (src/simulation.c:882-904), and the x velocities of that grid */
+static void maxwell_setup(int N, double L, const double *WMaxwL, const double *WMaxwR, double *vxM, double *WL, double *WR)
 {
-	const int N = q->N;
 	const size_t NN = (size_t)N * N, cell = 4 * NN;
 	const double scale = (2 * L) / N;
-	double *vxM = malloc(sizeof(double) * NN);
 	for (int l = 0; l < N; l++)
 		for (int k = 0; k < N; k++) vxM[k + (size_t)N * l] = l < N / 2 ? scale * (l + 1) : scale * (l - N);
-	double *WL = malloc(sizeof(double) * cell), *WR = malloc(sizeof(double) * cell);
 	for (int j = 0; j < 4; j++) {
 		to_maxwell_grid(N, &WMaxwL[j * NN], &WL[j * NN]);
 		to_maxwell_grid(N, &WMaxwR[j * NN], &WR[j * NN]);
@@ -455,6 +472,28 @@ void orc_InhomStepMaxwell(const orc_quad_t *q, unsigned numVol, double L, const 
 		WL[i] *= normL;
 		WR[i] *= normR;
 	}
+}
+
+/* one half transport (dt / 2) of the Maxwell step, as orc_InhomStepMaxwell applies it */
+void orc_TransportMaxwell(int N, unsigned numVol, double L, const double *W, double h, double dt, double lambda,
+	const double *WMaxwL, const double *WMaxwR, double *Wout)
+{
+	const size_t NN = (size_t)N * N, cell = 4 * NN;
+	double *vxM = malloc(sizeof(double) * NN), *WL = malloc(sizeof(double) * cell), *WR = malloc(sizeof(double) * cell);
+	maxwell_setup(N, L, WMaxwL, WMaxwR, vxM, WL, WR);
+	half_transport_maxwell(N, numVol, vxM, h, dt, lambda, WL, WR, W, Wout);
+	free(WR); free(WL); free(vxM);
+}
+
+/* Maxwell boundary step (src/simulation.c:882-1165): boundary states on the Maxwell grid,
+ * normalised by their incoming trace flux; half transport, collision of every cell, half transport */
+void orc_InhomStepMaxwell(const orc_quad_t *q, unsigned numVol, double L, const double *W, double h, double dt,
+	double lambda, const double *WMaxwL, const double *WMaxwR, const double *Bext, double *Wnext)
+{
+	const int N = q->N;
+	const size_t NN = (size_t)N * N, cell = 4 * NN;
+	double *vxM = malloc(sizeof(double) * NN), *WL = malloc(sizeof(double) * cell), *WR = malloc(sizeof(double) * cell);
+	maxwell_setup(N, L, WMaxwL, WMaxwR, vxM, WL, WR);
 	double *Wtmp = malloc(sizeof(double) * cell * numVol), *Wtmp2 = malloc(sizeof(double) * cell * numVol);
 	half_transport_maxwell(N, numVol, vxM, h, dt, lambda, WL, WR, W, Wtmp);
 	#pragma omp parallel for schedule(dynamic, 1)

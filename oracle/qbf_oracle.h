@@ -80,8 +80,12 @@ void orc_r2c_fill(int N, const double *in, double *out);
 void orc_c2r_real(int N, const double *in, double *out);
 void orc_InhomStepPeriodic(const orc_quad_t *q, unsigned numVol, double L, const double *W,
 	double h, double dt, const double *Bext, double *Wnext);
+void orc_InhomStepPeriodicCells(const orc_quad_t *q, unsigned numVol, double L, const double *W,
+	double h, double dt, const double *Bext, unsigned nlist, const unsigned *list, double *Wnext);
 void orc_InhomStepDirichlet(const orc_quad_t *q, unsigned numVol, double L, const double *W,
 	double h, double dt, const double *Bext, double *Wnext);
+void orc_TransportMaxwell(int N, unsigned numVol, double L, const double *W, double h, double dt, double lambda,
+	const double *WMaxwL, const double *WMaxwR, double *Wout);
 void orc_InhomStepMaxwell(const orc_quad_t *q, unsigned numVol, double L, const double *W, double h, double dt,
 	double lambda, const double *WMaxwL, const double *WMaxwR, const double *Bext, double *Wnext);
 void orc_SimulationHomogeneous(const orc_quad_t *q, const double *W0, double dt, unsigned numsteps,

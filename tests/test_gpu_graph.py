@@ -57,3 +57,53 @@ def test_capture_needs_a_fixed_pair_choice():
             e.capture_inhom_step_periodic(W, wl.L_BOX, 0.5, 1e-4)
     finally:
         e.close()
+
+
+def test_replays_are_ordered_with_the_current_stream():
+    """No synchronisation between eager work and replays: the input written on torch's stream just
+    before launch() is what the replay reads, and a read queued after launch() sees its result."""
+    N, Nx = 32, 5
+    L, h, dt = wl.L_BOX, 1.0 / Nx, wl.cfl_dt(N, Nx)
+    e = qbf.Engine(N, qbf.FP64, max_cells=Nx)
+    e.set_quadrature(*qbf.quadrature(N, 4, L, wl.R_BALL, 2))
+    states = [torch.from_numpy(wl.periodic(N, Nx, seed=s)).cuda() for s in range(3)]
+    refs = [e.inhom_step_periodic(s, L, h, dt) for s in states]
+    Win = states[0].clone()
+    g, out = e.capture_inhom_step_periodic(Win, L, h, dt)
+    torch.cuda.synchronize()
+    got = []
+    for k in (1, 2, 0):
+        Win.copy_(states[k])
+        g.launch()
+        got.append(out.clone())
+    torch.cuda.synchronize()
+    for k, x in zip((1, 2, 0), got):
+        assert torch.equal(x, refs[k]), k
+    g.close()
+    e.close()
+
+
+def test_a_live_graph_pins_the_engine_buffers():
+    """While a captured graph is alive the engine refuses to grow (reallocate) the buffers the graph
+    points to; after the graph is destroyed it grows them again.  An engine destroyed before its graph
+    leaves the graph dead: launch() raises instead of touching freed memory."""
+    N = 32
+    L, dt = wl.L_BOX, 1e-4
+    e = qbf.Engine(N, qbf.FP64, max_cells=8)
+    e.set_quadrature(*qbf.quadrature(N, 4, L, wl.R_BALL, 2))
+    W4 = torch.from_numpy(wl.periodic(N, 4, seed=1)).cuda()
+    W8 = torch.from_numpy(wl.periodic(N, 8, seed=1)).cuda()
+    g, out = e.capture_inhom_step_periodic(W4, L, 0.25, dt)
+    with pytest.raises(qbf.QBFError, match="captured graph"):
+        e.inhom_step_periodic(W8, L, 0.125, dt)
+    again = e.inhom_step_periodic(W4, L, 0.25, dt)      # no growth: allowed
+    g.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(out, again)
+    g.close()
+    e.inhom_step_periodic(W8, L, 0.125, dt)           # grows now
+    g2, _ = e.capture_inhom_step_periodic(W4, L, 0.25, dt)
+    e.close()
+    with pytest.raises(qbf.QBFError):
+        g2.launch()
+    g2.close()

@@ -218,3 +218,43 @@ def test_anchored_network_beats_network_free_fillers(setup):
     assert err["bare"] < 1.0
     assert err["anchored"] < 0.8 * err["frozen"]
     assert err["anchored"] < 0.5 * err["bare"]
+
+
+def test_non_finite_residual_falls_back(setup):
+    """A NaN in the offset field of one sampled cell makes the residual NaN: the policy must take the
+    exact step (a NaN compares false with the tolerance), and the state stays finite and equal to the
+    exact step's."""
+    qbf, S, wl, e, net, W, torch = setup
+    h, dt = 1.0 / 64, wl.cfl_dt(N, 64)
+    e.set_moment_correction(qbf.CONS_ALL)
+    ref = e.inhom_step_periodic(W, wl.L_BOX, h, dt)
+    st = S.SurrogateStepper(e, net, 2.0, 0.05, wl.L_BOX, h, dt, exact_every=0, sample_frac=1.0, tol=1.0)
+    D = torch.zeros(W.shape, dtype=torch.float32, device="cuda")
+    D[5, 2, 10, 10] = float("nan")
+    net.set_offset(D)
+    try:
+        out = st.step(W)
+    finally:
+        net.set_offset(None)
+    print("counts", st.counts, "residual", st.residuals)
+    assert st.counts["fallback"] == 1 and st.counts["surrogate"] == 0
+    assert not np.isfinite(st.residuals[-1])
+    assert bool(torch.isfinite(out).all())
+    assert torch.equal(out, ref)
+
+
+def test_offset_field_cell_count_is_enforced(setup):
+    """The offset field is tied to its cell count: stepping or checking another count is an error."""
+    qbf, S, wl, e, net, W, torch = setup
+    D = torch.zeros(W.shape, dtype=torch.float32, device="cuda")
+    net.set_offset(D)
+    try:
+        with pytest.raises(qbf.QBFError, match="offset field"):
+            net.collision_step(e, W[:10].contiguous(), 1e-3, 2.0, 0.05, wl.L_BOX)
+        with pytest.raises(qbf.QBFError, match="offset field"):
+            net.residual(e, W[:10].contiguous(), np.array([1, 2]), 2.0, 0.05)
+        with pytest.raises(qbf.QBFError):
+            net.set_offset(D.double())
+    finally:
+        net.set_offset(None)
+    net.collision_step(e, W[:10].contiguous(), 1e-3, 2.0, 0.05, wl.L_BOX)

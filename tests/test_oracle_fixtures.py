@@ -181,3 +181,50 @@ def test_oracle_inhom_step_maxwell_composition():
     assert np.array_equal(got, maxwell_transport(Wc, L, h, dt, lam, WL, WR))
     # the zero x-velocity row is not transported: it changes by the collision only
     assert np.array_equal(got[:, :, 0, :], Wc[:, :, 0, :])
+
+
+def _reflect(W):
+    """x -> 1 - x (cell order reversed) and v_x -> -v_x (row l -> (N - l) mod N) of states [.., 4, N, N]."""
+    N = W.shape[-1]
+    return W[..., (-np.arange(N)) % N, :][::-1].copy() if W.ndim == 4 else W[..., (-np.arange(N)) % N, :].copy()
+
+
+def test_oracle_maxwell_transport_commutes_with_reflection():
+    """Mirror symmetry of the Maxwell half transport, which pins its velocity reshuffle by
+    mathematics: reflecting the state (x -> 1 - x, v_x -> -v_x) and exchanging the two boundary
+    states (each reflected in v_x) gives the reflected result.  The Maxwell grid is symmetric
+    (x velocities +-dv ... +-L, src/simulation.c:155-161), the limiter is odd under A -> -A with the
+    cell order reversed, and the trace fluxes exchange sides.  The -L row (l = N/2) is excluded: the
+    Maxwell grid carries it twice, as -L and +L, and keeps the -L result, whose mirror image is the
+    discarded +L one.  A wrong reshuffle (a dropped or shifted row, a sign of the duplicated row)
+    breaks the symmetry by O(1)."""
+    N, L, h, dt, lam, nv = 8, 6.0, 0.08, 0.01, 0.35, 7
+    rng = np.random.default_rng(8)
+    W = 0.3 * rng.random((nv, 4, N, N)) + 0.1
+    WL, WR = 0.3 * rng.random((4, N, N)) + 0.1, 0.3 * rng.random((4, N, N)) + 0.1
+    a = O.transport_maxwell(W, L, h, dt, lam, WL, WR)
+    b = O.transport_maxwell(_reflect(W), L, h, dt, lam, _reflect(WR), _reflect(WL))
+    rows = [l for l in range(N) if l != N // 2]
+    err = np.abs(_reflect(a)[:, :, rows] - b[:, :, rows]).max() / np.abs(a).max()
+    print("reflection defect", err)
+    assert err < 1e-14
+    assert np.abs(a - W).max() > 1e-3 * np.abs(W).max()     # the transport moved something
+    # the step of the oracle applies this transport on both sides of the collision
+    q = O.Quad(N, 4, L, 3.75, 2)
+    Wc = np.stack([O.simulation_homogeneous(q, a[i], dt, 2)[1] for i in range(nv)])
+    assert np.array_equal(O.inhom_step_maxwell(q, W, h, dt, lam, WL, WR),
+                          O.transport_maxwell(Wc, L, h, dt, lam, WL, WR))
+
+
+def test_oracle_periodic_step_on_a_subset_of_cells():
+    """inhom_step_periodic_cells (collisions only within two cells of the listed ones) equals the full
+    periodic step on the listed cells, bit for bit (the limiter's stencil reaches two cells)."""
+    from conftest import rand_state
+    N, L, nv = 8, 6.0, 11
+    q = O.Quad(N, 4, L, 3.75, 2)
+    rng = np.random.default_rng(9)
+    W = rand_state(rng, nv, N, L)
+    full = O.inhom_step_periodic(q, W, 1.0 / nv, 0.004)
+    cells = [0, 5, 10]
+    part = O.inhom_step_periodic_cells(q, W, 1.0 / nv, 0.004, cells)
+    assert np.array_equal(full[cells], part[cells])
