@@ -27,7 +27,10 @@ extern "C" {
 
 typedef struct qbf_engine qbf_engine;
 
-enum { QBF_FP64 = 0, QBF_FP32 = 1 };              /* arithmetic of the convolution kernels */
+/* arithmetic of the convolution kernels.  QBF_BF16X3 (N = 64 only): as QBF_FP32, except that the
+ * line DFTs of the I3 K-operator (the dominant work) are GEMMs on the matrix cores with BF16 hi/lo
+ * split operands (hi*hi + hi*lo + lo*hi, FP32 accumulation); no counterpart in the reference. */
+enum { QBF_FP64 = 0, QBF_FP32 = 1, QBF_BF16X3 = 2 };
 enum { QBF_CC = 1, QBF_CD = 2, QBF_C = 3 };        /* which collision operator */
 enum { QBF_TAB_I1 = 0, QBF_TAB_I4 = 1 };           /* I1-type table slots */
 
@@ -130,6 +133,10 @@ int  qbf_graph_capture_inhom_step_periodic(qbf_engine *e, const double *dW, int 
 	double dt, const double *dBext, double *dOut, qbf_graph **g);
 int  qbf_graph_launch(qbf_graph *g);
 void qbf_graph_destroy(qbf_graph *g);
+/* The same for any sequence of calls on the engine's stream: fn(ctx) runs once eagerly, then once
+ * under capture (it must return 0; its calls may not allocate, copy between host and device or
+ * synchronise while captured; the engine's own entry points return -1 where they would). */
+int  qbf_graph_capture_calls(qbf_engine *e, int (*fn)(void *ctx), void *ctx, qbf_graph **g);
 
 /* numsteps frames of the periodic problem from dW0 (device), the state kept on the device
  * (src/simulation.c:1288-1301): frame 0 = W0, frame it = qbf_inhom_step_periodic of frame it-1
@@ -138,6 +145,15 @@ void qbf_graph_destroy(qbf_graph *g);
  * returns when they are there.  stride = 1 gives the reference's Wevolv. */
 int qbf_simulate_periodic(qbf_engine *e, const double *dW0, int ncell, double L, double h, double dt,
 	unsigned numsteps, unsigned stride, const double *dBext, double *hostFrames);
+
+/* The same loop with Dirichlet ends (frame it = qbf_inhom_step_dirichlet of frame it-1) and with
+ * Maxwell boundaries (qbf_inhom_step_maxwell; dWMaxwL, dWMaxwR as there), replacing the per-step
+ * host round trips of SimulationInhomogeneousDirichlet / Maxwell (src/simulation.c:1308-1341). */
+int qbf_simulate_dirichlet(qbf_engine *e, const double *dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double *dBext, double *hostFrames);
+int qbf_simulate_maxwell(qbf_engine *e, const double *dW0, int ncell, double L, double h, double dt, double lambda,
+	const double *dWMaxwL, const double *dWMaxwR, unsigned numsteps, unsigned stride, const double *dBext,
+	double *hostFrames);
 
 /* FP64 conservation correction of a collision step (physical states, [ncell][4][N][N]):
  * Wnew is replaced by Wold + P(Wnew - Wold), where P removes, per cell, the selected moments
@@ -243,6 +259,11 @@ int    qbf_surrogate_set_offset(qbf_cnn *c, const float *dD, int ncell);
 /* Strang step as qbf_inhom_step_periodic with the surrogate collision step; ncell <= maxBatch. */
 int    qbf_surrogate_inhom_step_periodic(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell, double L,
 	double h, double dt, float iscale, float oscale, int mask, double *dOut);
+/* that step captured as a HIP graph (qbf_graph_capture_calls; launch with qbf_graph_launch).  The
+ * graph keeps the network's weights and the offset field by address: later weight updates and an
+ * offset field rewritten in place are seen by replays; another offset array needs a new capture. */
+int    qbf_graph_capture_surrogate_inhom_step_periodic(qbf_cnn *c, qbf_engine *e, const double *dW, int ncell,
+	double L, double h, double dt, float iscale, float oscale, int mask, double *dOut, qbf_graph **g);
 
 /* device memory helpers for host code that does not include HIP headers */
 void *qbf_malloc(size_t bytes);

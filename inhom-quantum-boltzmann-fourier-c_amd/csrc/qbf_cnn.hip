@@ -876,4 +876,32 @@ int qbf_surrogate_inhom_step_periodic(qbf_cnn* c, qbf_engine* e, const double* d
 	return 0;
 }
 
+namespace {
+struct SurrogateStepCall {
+	qbf_cnn* c;
+	qbf_engine* e;
+	const double* W;
+	int ncell;
+	double L, h, dt;
+	float iscale, oscale;
+	int mask;
+	double* out;
+};
+int surrogate_step_call(void* p)
+{
+	auto* a = (SurrogateStepCall*)p;
+	return qbf_surrogate_inhom_step_periodic(a->c, a->e, a->W, a->ncell, a->L, a->h, a->dt, a->iscale, a->oscale,
+		a->mask, a->out);
+}
+} // namespace
+
+int qbf_graph_capture_surrogate_inhom_step_periodic(qbf_cnn* c, qbf_engine* e, const double* dW, int ncell, double L,
+	double h, double dt, float iscale, float oscale, int mask, double* dOut, qbf_graph** g)
+{
+	if (!c || !e || !dW || !dOut || !g) return cnn_err(-1, "qbf_graph_capture_surrogate_inhom_step_periodic: null argument");
+	SurrogateStepCall a{c, e, dW, ncell, L, h, dt, iscale, oscale, mask, dOut};
+	if (qbf_graph_capture_calls(e, surrogate_step_call, &a, g)) return cnn_err(-2, "%s", qbf_last_error());
+	return 0;
+}
+
 } // extern "C"

@@ -439,8 +439,15 @@ void SimulationInhomogeneousDirichlet(const wignerV_t *restrict_ W0, const quadI
 {
 	const unsigned nv = interm->numVol;
 	memcpy(Wevolv, W0, nv * sizeof(wignerV_t));
-	for (unsigned it = 1; it < numsteps; it++)
-		InhomStepDirichlet(&Wevolv[(size_t)(it - 1) * nv], quadI1, quadI2, quadI3, quadI4, h, dt, Bext, interm, &Wevolv[(size_t)it * nv]);
+	if (numsteps < 2) return;
+	if (!engine((int)nv) || use_all(quadI1, quadI2, quadI3, quadI4)) return;
+	const size_t sz = nv * sizeof(wignerV_t);
+	double *dA = dbuf(5, sz), *dB = dbuf(2, 3 * nv * sizeof(double));
+	if (!dA || !dB) return;
+	/* the state stays on the GPU; every frame is copied out (src/simulation.c:1308-1320) */
+	if (qbf_h2d(g_eng, dA, W0, sz) || qbf_h2d(g_eng, dB, Bext, 3 * nv * sizeof(double)) ||
+		qbf_simulate_dirichlet(g_eng, dA, (int)nv, interm->L, h, dt, numsteps, 1, dB, (double *)Wevolv))
+		report("SimulationInhomogeneousDirichlet");
 }
 
 void SimulationInhomogeneousMaxwell(const wignerV_t *restrict_ W0, const quadI1_t *restrict_ quadI1, const quadI2_t *restrict_ quadI2,
@@ -450,9 +457,17 @@ void SimulationInhomogeneousMaxwell(const wignerV_t *restrict_ W0, const quadI1_
 {
 	const unsigned nv = interm->numVol;
 	memcpy(Wevolv, W0, nv * sizeof(wignerV_t));
-	for (unsigned it = 1; it < numsteps; it++)
-		InhomStepMaxwell(&Wevolv[(size_t)(it - 1) * nv], quadI1, quadI2, quadI3, quadI4, h, dt, lambda, WMaxwL, WMaxwR,
-			Bext, interm, &Wevolv[(size_t)it * nv]);
+	if (numsteps < 2) return;
+	if (!engine((int)nv) || use_all(quadI1, quadI2, quadI3, quadI4)) return;
+	const size_t sz = nv * sizeof(wignerV_t);
+	double *dA = dbuf(5, sz), *dB = dbuf(2, 3 * nv * sizeof(double)), *dM = dbuf(6, 2 * sizeof(wignerV_t));
+	if (!dA || !dB || !dM) return;
+	double *dR = dM + sizeof(wignerV_t) / sizeof(double);
+	/* the state stays on the GPU; every frame is copied out (src/simulation.c:1327-1341) */
+	if (qbf_h2d(g_eng, dA, W0, sz) || qbf_h2d(g_eng, dB, Bext, 3 * nv * sizeof(double)) ||
+		qbf_h2d(g_eng, dM, WMaxwL, sizeof(wignerV_t)) || qbf_h2d(g_eng, dR, WMaxwR, sizeof(wignerV_t)) ||
+		qbf_simulate_maxwell(g_eng, dA, (int)nv, interm->L, h, dt, lambda, dM, dR, numsteps, 1, dB, (double *)Wevolv))
+		report("SimulationInhomogeneousMaxwell");
 }
 
 /* ------------------------------------------------------------------ finite_volume.h
@@ -609,19 +624,19 @@ int ReadData(const char *filename, void *data, const size_t size, const size_t n
 {
 	FILE *fd = fopen(filename, "rb");
 	if (!fd) {
-		fprintf(stderr, "'fopen()' failed during call of 'ReadData()'.\n");
+		fprintf(stderr, "ReadData: cannot open '%s'\n", filename);
 		return -1;
 	}
 	fseek(fd, 0, SEEK_END);
 	const long filesize = ftell(fd);
 	rewind(fd);
-	if ((size_t)filesize != n * size) {
-		fprintf(stderr, "'ReadData()' failed: expected file size does not match.\n");
+	if (filesize < 0 || (size_t)filesize != n * size) {
+		fprintf(stderr, "ReadData: '%s' holds %ld bytes, %zu expected\n", filename, filesize, n * size);
 		fclose(fd);
 		return -2;
 	}
 	if (fread(data, size, n, fd) != n) {
-		fprintf(stderr, "'fread()' failed during call of 'ReadData()'.\n");
+		fprintf(stderr, "ReadData: short read from '%s'\n", filename);
 		fclose(fd);
 		return -3;
 	}
@@ -633,11 +648,11 @@ int WriteData(const char *filename, const void *data, const size_t size, const s
 {
 	FILE *fd = fopen(filename, append ? "ab" : "wb");
 	if (!fd) {
-		fprintf(stderr, "'fopen()' failed during call of 'WriteData()'.\n");
+		fprintf(stderr, "WriteData: cannot open '%s'\n", filename);
 		return -1;
 	}
 	if (fwrite(data, size, n, fd) != n) {
-		fprintf(stderr, "'fwrite()' failed during call of 'WriteData()'.\n");
+		fprintf(stderr, "WriteData: short write to '%s'\n", filename);
 		fclose(fd);
 		return -3;
 	}

@@ -13,6 +13,7 @@
 // groups with Phi_i = sum_j psiR2_ij; the final convolution and Extract of every
 // integral are linear and are hoisted out of the term sums.
 #include "qbf_big.hip"
+#include "qbf_i3mfma.hip"
 #include "../../include/qbf.h"
 
 #include <math.h>
@@ -119,6 +120,7 @@ struct qbf_engine {
 	int prof_begin(hipEvent_t* b, hipEvent_t* e)
 	{
 		if (nev == evs.size()) {
+
 			hipEvent_t x, y;
 			if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) return -1;
 			evs.push_back({x, y});
@@ -203,6 +205,7 @@ struct DevBuf {
 	{
 		if (n <= bytes) return 0;
 		if (tl_capturing) return set_err(-1, "buffer growth during graph capture");
+
 		if (lock && *lock)
 			return set_err(-1, "buffer growth (%zu bytes) while a captured graph uses the engine's buffers: "
 				"destroy the graph first, or capture after a call of the largest size", n);
@@ -248,6 +251,10 @@ struct EngineT final : public qbf_engine {
 	I1Tab i1[2];
 	int n2 = 0; DevBuf i2p1, i2p2, i2pc;
 	int n3 = 0; DevBuf i3p1, i3p2n;
+	// BF16 hi/lo split path (precision QBF_BF16X3, N = 64): the I3 K-operator runs k_i3acc_mfma,
+	// whose psi2 table has its own order (i3p2m)
+	bool mfmaI3 = false;
+	DevBuf i3p2m;
 
 	DevBuf What, Ahat, Bhat, Sbuf, acc3, Fscr, xscr;
 	// Combination-term tables: each distinct table is uploaded once and its device copy reused, so
@@ -366,6 +373,23 @@ struct EngineT final : public qbf_engine {
 		int r = upload(i3p1, p1, sizeof(double2) * NN * (size_t)num);
 		if (!r) r = upload(i3p2n, p2n.data(), sizeof(R) * p2n.size());
 		if (r) return r;
+		if (mfmaI3) {
+			// k_i3acc_mfma's order: psi2 of row 16 w + i16, column 4 kq + r + 16 t at
+			// ((w * 8 + t) * 4 + r) * 64 + 16 kq + i16
+			static_assert(P == mf::P || N != 64, "the matrix-core kernel is built for P = 128");
+			std::vector<float> p2m((size_t)num * PP);
+			for (unsigned q = 0; q < num; q++)
+				for (int w = 0; w < 8; w++)
+					for (int t = 0; t < 8; t++)
+						for (int rr = 0; rr < 4; rr++)
+							for (int ln = 0; ln < 64; ln++) {
+								const int l = 16 * w + (ln & 15), k = 4 * (ln >> 4) + rr + 16 * t;
+								p2m[(size_t)q * PP + ((w * 8 + t) * 4 + rr) * 64 + ln] =
+									(float)p2[(size_t)q * PP + k + (size_t)P * l];
+							}
+			r = upload(i3p2m, p2m.data(), sizeof(float) * p2m.size());
+			if (r) return r;
+		}
 		n3 = (int)num;
 		return 0;
 	}
@@ -513,6 +537,18 @@ struct EngineT final : public qbf_engine {
 			if (prof && !tl_capturing && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
 			if constexpr (BIG)
 				hipLaunchKernelGGL((k_i3acc_big<R>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			else if (mfmaI3) {
+				if constexpr (std::is_same<R, float>::value && N == 64 && T == 1024) {
+					a.psi2n = i3p2m.p;
+					if (pair)
+						hipLaunchKernelGGL((k_i3acc_mfma<true>), dim3(ncell * nout * nchunk), dim3(mf::T), 0, stream, a);
+					else
+						hipLaunchKernelGGL((k_i3acc_mfma<false>), dim3(ncell * nout * nchunk), dim3(mf::T), 0, stream, a);
+				} else {
+					return set_err(-1, "the BF16 split path exists for N = 64 only");
+				}
+			} else if (std::is_same<R, double>::value && N == 64 && !pair)   // complex FP64 path at N = 64
+				hipLaunchKernelGGL(k_i3acc_d64, dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			else if (pair)
 				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, true>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			else
@@ -649,6 +685,7 @@ struct EngineT final : public qbf_engine {
 				return 0;
 			}
 		if (tl_capturing) return set_err(-1, "combination table not prepared: run the step once before capturing it");
+
 		if (termCache.size() >= TERM_CACHE && liveGraphs)
 			return set_err(-1, "combination-table cache full while a captured graph uses its tables: destroy the graph first");
 		if (termCache.size() >= TERM_CACHE) {
@@ -1034,11 +1071,15 @@ struct EngineT final : public qbf_engine {
 };
 
 template <typename R, int N>
-int make_engine(int maxCells, qbf_engine** out)
+int make_engine(int maxCells, qbf_engine** out, int precision)
 {
 	auto* e = new EngineT<R, N>();
 	e->gridN = N;
 	e->precision = std::is_same<R, float>::value ? QBF_FP32 : QBF_FP64;
+	if (precision == QBF_BF16X3) {
+		e->mfmaI3 = true;
+		e->precision = QBF_BF16X3;
+	}
 	e->maxCells = maxCells;
 	if (std::is_same<R, float>::value && N != 128) {
 		// the paired path deviates from the complex one by 0.15 (Nyquist content of smooth states) to
@@ -1067,14 +1108,15 @@ int qbf_engine_create(int N, int precision, int maxCells, qbf_engine** out)
 	if (maxCells < 1) return set_err(-1, "maxCells must be >= 1");
 	qbf_engine* e = nullptr;
 	int r;
-	if (precision == QBF_FP64 && N == 16) r = make_engine<double, 16>(maxCells, &e);
-	else if (precision == QBF_FP64 && N == 32) r = make_engine<double, 32>(maxCells, &e);
-	else if (precision == QBF_FP64 && N == 64) r = make_engine<double, 64>(maxCells, &e);
-	else if (precision == QBF_FP32 && N == 16) r = make_engine<float, 16>(maxCells, &e);
-	else if (precision == QBF_FP32 && N == 32) r = make_engine<float, 32>(maxCells, &e);
-	else if (precision == QBF_FP32 && N == 64) r = make_engine<float, 64>(maxCells, &e);
-	else if (precision == QBF_FP64 && N == 128) r = make_engine<double, 128>(maxCells, &e);
-	else if (precision == QBF_FP32 && N == 128) r = make_engine<float, 128>(maxCells, &e);
+	if (precision == QBF_FP64 && N == 16) r = make_engine<double, 16>(maxCells, &e, precision);
+	else if (precision == QBF_FP64 && N == 32) r = make_engine<double, 32>(maxCells, &e, precision);
+	else if (precision == QBF_FP64 && N == 64) r = make_engine<double, 64>(maxCells, &e, precision);
+	else if (precision == QBF_FP32 && N == 16) r = make_engine<float, 16>(maxCells, &e, precision);
+	else if (precision == QBF_FP32 && N == 32) r = make_engine<float, 32>(maxCells, &e, precision);
+	else if (precision == QBF_FP32 && N == 64) r = make_engine<float, 64>(maxCells, &e, precision);
+	else if (precision == QBF_BF16X3 && N == 64) r = make_engine<float, 64>(maxCells, &e, precision);
+	else if (precision == QBF_FP64 && N == 128) r = make_engine<double, 128>(maxCells, &e, precision);
+	else if (precision == QBF_FP32 && N == 128) r = make_engine<float, 128>(maxCells, &e, precision);
 	else return set_err(-1, "unsupported (N=%d, precision=%d)", N, precision);
 	if (r) return r;
 	hipStream_t s;
@@ -1252,29 +1294,26 @@ int qbf_last_collision(qbf_engine* e, int ncell, double* dWin, double* dC)
 // One Strang step (qbf_inhom_step_periodic with these buffers) as a HIP graph: the step runs once
 // eagerly (every buffer sized, every combination table uploaded), then is captured on the
 // engine's stream, which must not be the null stream.
-int qbf_graph_capture_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L, double h, double dt,
-	const double* Bext, double* out, qbf_graph** g)
+int qbf_graph_capture_calls(qbf_engine* e, int (*fn)(void*), void* ctx, qbf_graph** g)
 {
-	if (!e || !W || !out || !g) return set_err(-1, "qbf_graph_capture_inhom_step_periodic: null argument");
+	if (!e || !fn || !g) return set_err(-1, "qbf_graph_capture_calls: null argument");
 	*g = nullptr;
-	if (!e->stream) return set_err(-1, "qbf_graph_capture_inhom_step_periodic: the null stream cannot be captured");
-	if (e->pairMode == 1 && e->gridN != 128)
-		return set_err(-1, "qbf_graph_capture_inhom_step_periodic: needs a fixed paired-path choice "
-			"(qbf_set_real_pairs mode 0 or 2)");
-	int r = e->inhom_step(W, ncell, L, h, dt, Bext, out);
+	if (!e->stream) return set_err(-1, "qbf_graph_capture_calls: the null stream cannot be captured");
+	// once eagerly: sizes every buffer and uploads every combination table the calls need
+	int r = fn(ctx);
 	if (r) return r;
 	HIPCHK(hipStreamSynchronize(e->stream));
 	const bool prof = e->prof;
 	e->prof = false;
 	HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
 	tl_capturing = true;
-	r = e->inhom_step(W, ncell, L, h, dt, Bext, out);
+	r = fn(ctx);
 	tl_capturing = false;
 	e->prof = prof;
 	hipGraph_t graph = nullptr;
 	const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
 	if (r || ce != hipSuccess) {
-		if (graph) hipGraphDestroy(graph);
+		if (graph) (void)hipGraphDestroy(graph);
 		return r ? r : set_err(-2, "hipStreamEndCapture failed: %s", hipGetErrorString(ce));
 	}
 	auto* ng = new qbf_graph();
@@ -1297,6 +1336,32 @@ int qbf_graph_capture_inhom_step_periodic(qbf_engine* e, const double* W, int nc
 	e->graphs.push_back(ng);
 	*g = ng;
 	return 0;
+}
+
+namespace {
+struct StepCall {
+	qbf_engine* e;
+	const double *W, *Bext;
+	int ncell;
+	double L, h, dt;
+	double* out;
+};
+int step_call(void* p)
+{
+	auto* c = (StepCall*)p;
+	return c->e->inhom_step(c->W, c->ncell, c->L, c->h, c->dt, c->Bext, c->out);
+}
+} // namespace
+
+int qbf_graph_capture_inhom_step_periodic(qbf_engine* e, const double* W, int ncell, double L, double h, double dt,
+	const double* Bext, double* out, qbf_graph** g)
+{
+	if (!e || !W || !out || !g) return set_err(-1, "qbf_graph_capture_inhom_step_periodic: null argument");
+	if (e->pairMode == 1 && e->gridN != 128)
+		return set_err(-1, "qbf_graph_capture_inhom_step_periodic: needs a fixed paired-path choice "
+			"(qbf_set_real_pairs mode 0 or 2)");
+	StepCall c{e, W, Bext, ncell, L, h, dt, out};
+	return qbf_graph_capture_calls(e, step_call, &c, g);
 }
 
 // A replay runs on the capture stream, after the work already queued on the engine's current stream
@@ -1370,37 +1435,89 @@ int qbf_set_moment_correction(qbf_engine* e, int mask)
 	return 0;
 }
 
-int qbf_simulate_periodic(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
-	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames)
+// boundary of a device-resident simulation loop: periodic, Dirichlet or Maxwell (lambda and the two
+// boundary states, device)
+struct Boundary {
+	int kind = 0;                     // 0 periodic, 1 Dirichlet, 2 Maxwell
+	double lambda = 0.0;
+	const double* WL = nullptr;
+	const double* WR = nullptr;
+};
+
+// the device-resident loop of qbf_simulate_{periodic,dirichlet,maxwell} over two state buffers
+static int simulate_loop(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames, double* const buf[2], size_t bytes,
+	const Boundary& bc = Boundary())
 {
-	if (!e || !dW0 || !hostFrames || ncell < 1 || stride < 1) return set_err(-1, "qbf_simulate_periodic: bad argument");
-	const size_t bytes = sizeof(double) * 4 * (size_t)e->gridN * e->gridN * ncell;
-	double* buf[2] = {nullptr, nullptr};
-	if (hipMalloc(&buf[0], bytes) != hipSuccess || hipMalloc(&buf[1], bytes) != hipSuccess) {
-		if (buf[0]) hipFree(buf[0]);
-		return set_err(-3, "qbf_simulate_periodic: hipMalloc(%zu) failed", bytes);
-	}
 	int r = 0;
 	const double* cur = dW0;
 	size_t f = 0;
 	// frame 0 is the initial state (src/simulation.c:1294)
 	if (hipMemcpyAsync(hostFrames, dW0, bytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess)
-		r = set_err(-2, "qbf_simulate_periodic: copy of frame 0 failed");
+		r = set_err(-2, "qbf_simulate: copy of frame 0 failed");
 	f = 1;
 	for (unsigned it = 1; it < numsteps && !r; it++) {
 		double* nxt = buf[it & 1];
-		r = e->inhom_step(cur, ncell, L, h, dt, dBext, nxt);
+		if (bc.kind == 1)
+			r = e->inhom_step_dirichlet(cur, ncell, L, h, dt, dBext, nxt);
+		else if (bc.kind == 2)
+			r = e->inhom_step_maxwell(cur, ncell, L, h, dt, bc.lambda, bc.WL, bc.WR, dBext, nxt);
+		else
+			r = e->inhom_step(cur, ncell, L, h, dt, dBext, nxt);
 		cur = nxt;
 		if (!r && it % stride == 0) {
 			if (hipMemcpyAsync((char*)hostFrames + f * bytes, cur, bytes, hipMemcpyDeviceToHost, e->stream) != hipSuccess)
-				r = set_err(-2, "qbf_simulate_periodic: frame copy failed");
+				r = set_err(-2, "qbf_simulate: frame copy failed");
 			f++;
 		}
 	}
-	if (hipStreamSynchronize(e->stream) != hipSuccess && !r) r = set_err(-2, "qbf_simulate_periodic: stream error");
-	hipFree(buf[0]);
-	hipFree(buf[1]);
+	if (hipStreamSynchronize(e->stream) != hipSuccess && !r) r = set_err(-2, "qbf_simulate: stream error");
 	return r;
+}
+
+// two state buffers of hipMalloc'ed device memory around simulate_loop
+static int simulate_alloc(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames, const Boundary& bc, const char* who)
+{
+	if (!e || !dW0 || !hostFrames || ncell < 1 || stride < 1) return set_err(-1, "%s: bad argument", who);
+	const size_t bytes = sizeof(double) * 4 * (size_t)e->gridN * e->gridN * ncell;
+	double* buf[2] = {nullptr, nullptr};
+	if (hipMalloc(&buf[0], bytes) != hipSuccess || hipMalloc(&buf[1], bytes) != hipSuccess) {
+		if (buf[0]) (void)hipFree(buf[0]);
+		return set_err(-3, "%s: hipMalloc(%zu) failed", who, bytes);
+	}
+	const int r = simulate_loop(e, dW0, ncell, L, h, dt, numsteps, stride, dBext, hostFrames, buf, bytes, bc);
+	(void)hipFree(buf[0]);
+	(void)hipFree(buf[1]);
+	return r;
+}
+
+int qbf_simulate_periodic(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames)
+{
+	return simulate_alloc(e, dW0, ncell, L, h, dt, numsteps, stride, dBext, hostFrames, Boundary(),
+		"qbf_simulate_periodic");
+}
+
+int qbf_simulate_dirichlet(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames)
+{
+	Boundary bc;
+	bc.kind = 1;
+	return simulate_alloc(e, dW0, ncell, L, h, dt, numsteps, stride, dBext, hostFrames, bc, "qbf_simulate_dirichlet");
+}
+
+int qbf_simulate_maxwell(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt, double lambda,
+	const double* dWMaxwL, const double* dWMaxwR, unsigned numsteps, unsigned stride, const double* dBext,
+	double* hostFrames)
+{
+	if (!dWMaxwL || !dWMaxwR) return set_err(-1, "qbf_simulate_maxwell: null boundary state");
+	Boundary bc;
+	bc.kind = 2;
+	bc.lambda = lambda;
+	bc.WL = dWMaxwL;
+	bc.WR = dWMaxwR;
+	return simulate_alloc(e, dW0, ncell, L, h, dt, numsteps, stride, dBext, hostFrames, bc, "qbf_simulate_maxwell");
 }
 
 int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, double* dOut)

@@ -21,7 +21,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # QBF_LIB_PATH selects another build of the library (for example one with other kernel shapes)
 LIB_PATH = os.environ.get("QBF_LIB_PATH") or os.path.join(PKG_DIR, "lib", "libqbf.so")
 
-FP64, FP32 = 0, 1
+FP64, FP32, BF16X3 = 0, 1, 2
 CC, CD, CFULL = 1, 2, 3
 TAB_I1, TAB_I4 = 0, 1
 CONS_MASS, CONS_MOMENTUM, CONS_ENERGY = 1, 2, 4
@@ -94,6 +94,8 @@ def lib():
             "qbf_fd_distance": (i, [vp, vp, i, d, vp, i, vp]),
             "qbf_cell_average": (i, [vp, vp, i, vp]),
             "qbf_simulate_periodic": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
+            "qbf_simulate_dirichlet": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
+            "qbf_simulate_maxwell": (i, [vp, vp, i, d, d, d, d, vp, vp, u, u, vp, vp]),
             "qbf_profile": (i, [vp, i]),
             "qbf_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i), C.POINTER(d)]),
             "qbf_legendre_gauss": (None, [u, d, d, _dp, _dp]),
@@ -234,8 +236,26 @@ class Engine:
         nf = (numsteps - 1) // stride + 1 if numsteps > 0 else 0
         out = np.empty((max(nf, 1),) + tuple(W0.shape))
         _check(lib().qbf_simulate_periodic(self._h, self._R(W0), W0.shape[0], L, h, dt, numsteps, stride,
-                                           self._B(Bext, W0.shape[0]),
-                                           out.ctypes.data_as(C.c_void_p)))
+                                           self._B(Bext, W0.shape[0]), out.ctypes.data_as(C.c_void_p)))
+        return out[:nf]
+
+    def simulate_dirichlet(self, W0, L, h, dt, numsteps, stride=1, Bext=None):
+        """As simulate_periodic with Dirichlet ends (qbf_inhom_step_dirichlet per step)."""
+        self._bind()
+        nf = (numsteps - 1) // stride + 1 if numsteps > 0 else 0
+        out = np.empty((max(nf, 1),) + tuple(W0.shape))
+        _check(lib().qbf_simulate_dirichlet(self._h, self._R(W0), W0.shape[0], L, h, dt, numsteps, stride,
+                                            self._B(Bext, W0.shape[0]), out.ctypes.data_as(C.c_void_p)))
+        return out[:nf]
+
+    def simulate_maxwell(self, W0, L, h, dt, lam, WL, WR, numsteps, stride=1, Bext=None):
+        """As simulate_periodic with Maxwell boundaries (qbf_inhom_step_maxwell per step)."""
+        self._bind()
+        nf = (numsteps - 1) // stride + 1 if numsteps > 0 else 0
+        out = np.empty((max(nf, 1),) + tuple(W0.shape))
+        _check(lib().qbf_simulate_maxwell(self._h, self._R(W0), W0.shape[0], L, h, dt, lam, self._R1(WL, "WL"),
+                                          self._R1(WR, "WR"), numsteps, stride, self._B(Bext, W0.shape[0]),
+                                          out.ctypes.data_as(C.c_void_p)))
         return out[:nf]
 
     def cell_diagnostics(self, W, L):

@@ -41,6 +41,8 @@ def _lib():
             "qbf_surrogate_inhom_step_periodic": (i, [vp, vp, vp, i, C.c_double, C.c_double, C.c_double, f, f, i,
                                                       vp]),
             "qbf_surrogate_set_offset": (i, [vp, vp, i]),
+            "qbf_graph_capture_surrogate_inhom_step_periodic": (i, [vp, vp, vp, i, C.c_double, C.c_double, C.c_double,
+                                                                    f, f, i, vp, C.POINTER(vp)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -187,6 +189,23 @@ class CNN:
         _check(_lib().qbf_surrogate_inhom_step_periodic(self._h, engine._h, _ptrd(W), W.shape[0], L, h, dt, iscale,
                                                         oscale, mask, _ptrd(out)))
         return out
+
+    def capture_inhom_step_periodic(self, engine, W, L, h, dt, iscale, oscale, mask=0, out=None):
+        """HIP graph of one surrogate Strang step from the buffer W into the buffer out (returned with
+        the graph), as Engine.capture_inhom_step_periodic: the step runs once eagerly first, capture
+        and replays use a dedicated torch stream.  The graph keeps this network's weights and its
+        offset field by address (an offset field must then be updated in place)."""
+        from . import Graph
+        torch = self._torch
+        out = torch.empty_like(W) if out is None else out
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        g = C.c_void_p()
+        with torch.cuda.stream(stream):
+            engine._bind()
+            _check(_lib().qbf_graph_capture_surrogate_inhom_step_periodic(
+                self._h, engine._h, _ptrd(W), W.shape[0], L, h, dt, iscale, oscale, mask, _ptrd(out), C.byref(g)))
+        return Graph(g, stream, (W, out, self._offset, self), engine), out
 
 
 def _ptrd(t):

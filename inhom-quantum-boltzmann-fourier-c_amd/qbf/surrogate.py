@@ -2,8 +2,8 @@
 and the stepping policy that keeps it honest.
 
 Training data are physical states of the periodic workload family (qbf.workloads.periodic)
-at several free-streaming times, with the exact collision operator C = Cc + Cd of the FP32
-engine as target.  The network learns C(W) ~ oscale * net(iscale * W) with Adam on its own
+sampled along exact-solver trajectories (trajectory_set), or at several free-streaming times
+(training_set), with the exact collision operator C = Cc + Cd of the FP32 engine as target.  The network learns C(W) ~ oscale * net(iscale * W) with Adam on its own
 GPU kernels (qbf.cnn).
 
 Stepping policy (SurrogateStepper.step), per Strang step of the periodic problem:
@@ -82,6 +82,37 @@ def training_set(engine, seeds, times, cells_per_state=64, Nx=512, chunk=None, r
         Y.append(exact_collision_phys(engine, Xd[i:i + chunk].contiguous()))
     Yd = torch.cat(Y)
     return Xd.float().contiguous(), Yd.float().contiguous()
+
+
+def trajectory_set(engine, seeds, t_end=0.04, Nx=32, per_seed=None, rng_seed=0, kmax=4):
+    """(X, Y): float32 cuda tensors [n, 4, N, N] of states sampled along exact-solver trajectories
+    and their exact collision operator.  For each seed the periodic workload on Nx cells (h = 1/Nx,
+    CFL 0.5) is stepped with the exact engine (its precision and conservation setting) until t_end;
+    every step's collision yields pairs for all its cells: the state after the first half transport
+    and the operator there (Engine.last_collision, taken from the step itself: no extra evaluation).
+    per_seed: pairs sampled per seed (all if None)."""
+    import torch
+    rng = np.random.default_rng(rng_seed)
+    h, dt = 1.0 / Nx, wl.cfl_dt(engine.N, Nx)
+    nsteps = max(1, int(round(t_end / dt)))
+    X, Y = [], []
+    for s in seeds:
+        W = torch.from_numpy(wl.periodic(engine.N, Nx, seed=int(s), kmax=kmax)).cuda()
+        xs, ys = [], []
+        buf = torch.empty_like(W)
+        for _ in range(nsteps):
+            engine.inhom_step_periodic(W, wl.L_BOX, h, dt, out=buf)
+            Win, C = engine.last_collision(Nx)
+            xs.append(Win.float())
+            ys.append(C.float())
+            W, buf = buf, W
+        xs, ys = torch.cat(xs), torch.cat(ys)
+        if per_seed is not None and per_seed < xs.shape[0]:
+            idx = torch.from_numpy(np.sort(rng.choice(xs.shape[0], per_seed, replace=False))).cuda()
+            xs, ys = xs.index_select(0, idx), ys.index_select(0, idx)
+        X.append(xs)
+        Y.append(ys)
+    return torch.cat(X).contiguous(), torch.cat(Y).contiguous()
 
 
 def scales(X, Y, norm=False):

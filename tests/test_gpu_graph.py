@@ -1,4 +1,5 @@
-"""HIP-graph capture of the periodic Strang step (qbf_graph_capture_inhom_step_periodic).  Every
+"""HIP-graph capture of the periodic Strang step (qbf_graph_capture_inhom_step_periodic) and of the
+surrogate Strang step (qbf_graph_capture_surrogate_inhom_step_periodic).  Every
 replay, with the program's own eager work and synchronisations between the replays, must equal the
 eager step on the same input bit for bit (the same kernels in the same order; the engine's kernels
 have no atomics).  The step path records no memset nodes: the accumulators are cleared by the
@@ -107,3 +108,41 @@ def test_a_live_graph_pins_the_engine_buffers():
     with pytest.raises(qbf.QBFError):
         g2.launch()
     g2.close()
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_surrogate_step_graph_replays_equal_eager_steps(N):
+    """The surrogate Strang step (qbf_graph_capture_surrogate_inhom_step_periodic): replays equal eager
+    steps bit for bit, and an offset field rewritten in place between replays is what a replay uses."""
+    from qbf.cnn import CNN
+    Nx = 6
+    L, h, dt = wl.L_BOX, 1.0 / Nx, wl.cfl_dt(N, Nx)
+    print(f"surrogate graph N={N}", flush=True)
+    e = qbf.Engine(N, qbf.FP32, max_cells=Nx)
+    e.set_quadrature(*qbf.quadrature(N, 4, L, wl.R_BALL, 2))
+    net = CNN(max_batch=Nx, seed=3, N=N)
+    rng = np.random.default_rng(N)
+    D = torch.from_numpy((1e-3 * rng.standard_normal((Nx, 4, N, N))).astype(np.float32)).cuda()
+    net.set_offset(D)
+    states = [torch.from_numpy(wl.periodic(N, Nx, seed=s)).cuda() for s in range(3)]
+    isc, osc = 2.0, 0.05
+    Win = states[0].clone()
+    g, out = net.capture_inhom_step_periodic(e, Win, L, h, dt, isc, osc, mask=qbf.CONS_ALL)
+    torch.cuda.synchronize()
+    ref = net.inhom_step_periodic(e, states[0], L, h, dt, isc, osc, mask=qbf.CONS_ALL)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    for k, scale in ((1, 1.0), (2, -2.0), (1, 0.5)):
+        Win.copy_(states[k])
+        D.mul_(scale)                               # the offset field rewritten in place
+        out.zero_()
+        torch.cuda.synchronize()
+        g.launch()
+        g.stream.synchronize()
+        ref = net.inhom_step_periodic(e, states[k], L, h, dt, isc, osc, mask=qbf.CONS_ALL)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), k
+        assert not torch.equal(out, states[k])
+    g.close()
+    net.close()
+    e.close()
