@@ -547,9 +547,7 @@ struct EngineT final : public qbf_engine {
 				} else {
 					return set_err(-1, "the BF16 split path exists for N = 64 only");
 				}
-			} else if (std::is_same<R, double>::value && N == 64 && !pair)   // complex FP64 path at N = 64
-				hipLaunchKernelGGL(k_i3acc_d64, dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
-			else if (pair)
+			} else if (pair)
 				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, true>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
 			else
 				hipLaunchKernelGGL((k_i3acc<R, N, T, XG, false>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);

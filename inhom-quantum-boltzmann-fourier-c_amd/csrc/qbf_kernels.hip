@@ -336,162 +336,6 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP64 at N = 64 (P = 128, T = 1024: G = 8 lanes per line, E = 16 elements each).  A padded FP64
-// grid (256 KiB) and its accumulator fill the whole register file of a CU (512 KiB), and the
-// grid does not fit in LDS (160 KiB).  So this variant keeps the accumulator in global memory
-// (read, updated and written once per K-operator application: L2/MALL traffic instead of register
-// spills) and exchanges rows and columns through LDS in two halves of 64 columns, with 8 elements
-// of register staging per thread:
-//   R_i -> C_b  pass h: every thread writes its elements of columns [64h, 64h + 64) (s in [8h, 8h + 8));
-//               the threads of those columns (t in [512h, 512h + 512)) read all 16 of theirs
-//               (pass 0: 8 into the slots just written, 8 into the staging registers).
-//   C_b -> R_i  pass h: the threads of columns [64h, 64h + 64) write all 16; every thread reads its
-//               8 elements of those columns (the threads that write in pass 1 stage theirs in pass 0).
-// Same arithmetic, operand order and results as k_i3acc<double, 64, 1024, true, false>.
-struct XHalf64 {
-	// LDS grid of 128 rows x 64 columns, 16-byte elements: pitch 72, one element of skew per block of
-	// 16 rows (an 8-lane phase of a column read covers 8 row blocks: distinct banks)
-	static constexpr int PITCH = 72, SIZE = 128 * PITCH + 8;
-	double2* g;
-	__device__ __forceinline__ int a(int row, int col) const { return row * PITCH + col + (row >> 4); }
-};
-
-template <bool RI_TO_CB>
-__device__ __forceinline__ void xchg_half64(double2* v, double2* stage, const XHalf64& X, int line, int p)
-{
-	constexpr int G = 8, E = 16;
-	const int t = threadIdx.x;
-	double2* g = X.g + opaque_zero();
-	const int r0 = E * brev<G>(p);
-	if constexpr (RI_TO_CB) {
-		// pass 0: columns 0..63
-		__syncthreads();
-#pragma unroll
-		for (int s = 0; s < 8; s++) g[X.a(line, p + G * s)] = v[s];
-		__syncthreads();
-		if (t < 512) {
-#pragma unroll
-			for (int u = 0; u < 8; u++) v[u] = g[X.a(r0 + u, line)];
-#pragma unroll
-			for (int u = 0; u < 8; u++) stage[u] = g[X.a(r0 + 8 + u, line)];
-		}
-		// pass 1: columns 64..127
-		__syncthreads();
-#pragma unroll
-		for (int s = 8; s < 16; s++) g[X.a(line, p + G * s - 64)] = v[s];
-		__syncthreads();
-		if (t < 512) {
-#pragma unroll
-			for (int u = 0; u < 8; u++) v[8 + u] = stage[u];
-		} else {
-#pragma unroll
-			for (int u = 0; u < E; u++) v[u] = g[X.a(r0 + u, line - 64)];
-		}
-	} else {
-		// pass 0: the threads of columns 0..63 write; everyone reads elements s < 8
-		__syncthreads();
-		if (t < 512) {
-#pragma unroll
-			for (int u = 0; u < E; u++) g[X.a(r0 + u, line)] = v[u];
-		}
-		__syncthreads();
-		if (t < 512) {
-#pragma unroll
-			for (int s = 0; s < 8; s++) v[s] = g[X.a(line, p + G * s)];
-		} else {
-#pragma unroll
-			for (int s = 0; s < 8; s++) stage[s] = g[X.a(line, p + G * s)];
-		}
-		// pass 1: the threads of columns 64..127 write; everyone reads elements s >= 8
-		__syncthreads();
-		if (t >= 512) {
-#pragma unroll
-			for (int u = 0; u < E; u++) g[X.a(r0 + u, line - 64)] = v[u];
-		}
-		__syncthreads();
-#pragma unroll
-		for (int s = 8; s < 16; s++) v[s] = g[X.a(line, p + G * s - 64)];
-		if (t >= 512) {
-#pragma unroll
-			for (int s = 0; s < 8; s++) v[s] = stage[s];
-		}
-	}
-}
-
-__global__ __launch_bounds__(1024) void k_i3acc_d64(I3Args a)
-{
-	using R = double;
-	constexpr int N = 64, T = 1024;
-	using GE = Geo<2 * N, T>;
-	constexpr int P = GE::P, E = GE::E, G = GE::G;
-	static_assert(G == 8 && E == 16, "geometry of the half exchanges");
-	__shared__ double2 lds[XHalf64::SIZE];
-	__shared__ double2 tw[TwTab<P, T>::SIZE];
-	load_twiddles<R, P, T>(tw, a.tw256);
-	__syncthreads();
-	const int lb = xcd_block();
-	const int chunk = lb % a.nchunk;
-	const int out = (lb / a.nchunk) % a.nout;
-	const int cell = lb / (a.nchunk * a.nout);
-	const int line = threadIdx.x / G, p = threadIdx.x % G;
-	const int t = threadIdx.x;
-	const int qa = (int)(((long long)chunk * a.nq) / a.nchunk);
-	const int qb = (int)(((long long)(chunk + 1) * a.nq) / a.nchunk);
-	double2* accp = (double2*)a.acc + (((size_t)cell * a.nout + out) * a.nchunk + chunk) * (P * P);
-	const XHalf64 X{lds};
-	double2 v[E], stage[8];
-	const double2* Sh = (const double2*)a.Shat + (size_t)cell * a.nS * (P * P);
-	const int ni = a.ninst[out];
-	bool first = !a.accumulate;
-	for (int qq = qa; qq < qb; qq++) {
-		const double2* Fq = (const double2*)a.F + ((size_t)cell * a.nq + qq) * a.nF * (P * P);
-		const double* ps = (const double*)a.psi2n + (size_t)(a.q0 + qq) * (P * P);
-		for (int in = 0; in < ni; in++) {
-			const double2* Sg = Sh + (size_t)a.instS[out][in] * (P * P);
-			const double2* Fh = Fq + (size_t)a.instH[out][in] * (P * P);
-			const double2* Ff = Fq + (size_t)a.instF[out][in] * (P * P);
-#pragma unroll
-			for (int u = 0; u < E; u++) v[u] = cmul<R>(uni(Sg, (long long)u * T)[t], uni(Fh, (long long)u * T)[t]);
-			// IFFT2: rows (R_b -> R_i), exchange, columns (C_b -> C_i)
-			line_bwd<R, P, G, E>(v, p, tw);
-			xchg_half64<true>(v, stage, X, line, p);
-			line_bwd<R, P, G, E>(v, p, tw);
-			__builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-			for (int s = 0; s < E; s++) {
-				const R m = uni(ps, (long long)s * T)[t];
-				v[s].x *= m;
-				v[s].y *= m;
-			}
-			// FFT2: columns (C_i -> C_b), exchange, rows (R_i -> R_b)
-			line_fwd<R, P, G, E>(v, p, tw);
-			xchg_half64<false>(v, stage, X, line, p);
-			line_fwd<R, P, G, E>(v, p, tw);
-			__builtin_amdgcn_sched_barrier(0);
-			const R w = (R)a.instW[out][in];
-#pragma unroll
-			for (int u = 0; u < E; u++) {
-				const double2 y = cmul<R>(uni(Ff, (long long)u * T)[t], v[u]);
-				double2* ap = uni(accp, (long long)u * T) + t;
-				if (first) {
-					*ap = make_double2(w * y.x, w * y.y);
-				} else {
-					const double2 c = *ap;
-					*ap = make_double2(c.x + w * y.x, c.y + w * y.y);
-				}
-			}
-			first = false;
-			__builtin_amdgcn_sched_barrier(0);
-		}
-	}
-	if (first) {
-		// an empty term range leaves a zero partial sum (as k_i3acc stores its zero accumulator)
-#pragma unroll
-		for (int u = 0; u < E; u++) uni(accp, (long long)u * T)[t] = make_double2(0.0, 0.0);
-	}
-}
-
-// ---------------------------------------------------------------------------------------------
 template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
 {
