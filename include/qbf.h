@@ -155,6 +155,20 @@ int qbf_simulate_maxwell(qbf_engine *e, const double *dW0, int ncell, double L, 
 	const double *dWMaxwL, const double *dWMaxwR, unsigned numsteps, unsigned stride, const double *dBext,
 	double *hostFrames);
 
+/* qbf_simulate_periodic without any device allocation; no counterpart in the reference.  For callers
+ * that cannot allocate (tight stepping loops, code that must not call hipMalloc).  The two state
+ * buffers of the loop come from a static device array of 2 * QBF_NOALLOC_STATE_BYTES (64 MiB) that
+ * each device holds from the library's first use on it until the process ends, so a state may hold
+ * at most QBF_NOALLOC_STATE_BYTES (ncell * 4 * N * N * 8 bytes: 1024 cells at N = 32, 256 at
+ * N = 64, 64 at N = 128; larger states return -1).  The engine's own buffers and combination
+ * tables must already be sized for ncell cells (one qbf_inhom_step_periodic or
+ * qbf_simulate_periodic call on that many cells does it); otherwise the call returns -1 instead of
+ * allocating.  Calls on the engine's device (the current device) share the array: calls that
+ * overlap on one device race.  Results equal qbf_simulate_periodic's bit for bit. */
+#define QBF_NOALLOC_STATE_BYTES (32u << 20)
+int qbf_simulate_periodic_noalloc(qbf_engine *e, const double *dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double *dBext, double *hostFrames);
+
 /* FP64 conservation correction of a collision step (physical states, [ncell][4][N][N]):
  * Wnew is replaced by Wold + P(Wnew - Wold), where P removes, per cell, the selected moments
  * of the increment with a |w0|-weighted least-squares correction.  mask: QBF_CONS_MASS (mass

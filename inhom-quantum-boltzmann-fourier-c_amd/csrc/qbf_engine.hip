@@ -21,6 +21,7 @@
 #include <stdarg.h>
 #include <string.h>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -52,6 +53,39 @@ extern "C" const char* qbf_last_error(void) { return g_err.c_str(); }
 // set while a step is captured into a HIP graph (qbf_graph_capture_inhom_step_periodic): no
 // allocation, host copy, read-back or synchronisation may happen then
 static thread_local bool tl_capturing = false;
+// set inside qbf_simulate_periodic_noalloc: a buffer that would have to grow, a combination table
+// not yet on the device or a profiling event that would have to be created is an error instead
+static thread_local bool tl_noalloc = false;
+
+// The two state buffers of qbf_simulate_periodic_noalloc.  Each device holds its own copy of this
+// array (allocated when the code object is loaded there, kept until the process ends); host code
+// uses only the address that hipGetSymbolAddress returns for the device (g_scratch_dev), never the
+// array's name.  Calls on one device share it, so calls that overlap on one device race.
+static constexpr size_t QBF_NOALLOC_SCRATCH_BYTES = (size_t)QBF_NOALLOC_STATE_BYTES * 2;
+static __device__ __align__(256) unsigned char scratch[QBF_NOALLOC_SCRATCH_BYTES];
+static constexpr int QBF_MAX_DEVICES = 64;
+static void* g_scratch_dev[QBF_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the current device and its scratch address (looked up once per device)
+static int scratch_for_current_device(int* devOut, void** ptrOut)
+{
+	int dev = -1;
+	hipError_t st = hipGetDevice(&dev);
+	if (st != hipSuccess || dev < 0 || dev >= QBF_MAX_DEVICES)
+		return set_err(-2, "hipGetDevice failed or device %d beyond %d", dev, QBF_MAX_DEVICES);
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_dev[dev]) {
+		void* p = nullptr;
+		st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (st != hipSuccess || !p)
+			return set_err(-2, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(st));
+		g_scratch_dev[dev] = p;
+	}
+	*devOut = dev;
+	*ptrOut = g_scratch_dev[dev];
+	return 0;
+}
 
 // threads per workgroup for a padded P x P grid (P = 2N), and for an unpadded N x N grid
 // XG: exchange rows and columns through a per-workgroup global scratch slot instead of LDS
@@ -120,6 +154,7 @@ struct qbf_engine {
 	int prof_begin(hipEvent_t* b, hipEvent_t* e)
 	{
 		if (nev == evs.size()) {
+			if (tl_noalloc) return -1;
 
 			hipEvent_t x, y;
 			if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) return -1;
@@ -132,6 +167,9 @@ struct qbf_engine {
 	}
 	hipStream_t stream = nullptr;
 	bool ownStream = false;
+	// the device the engine was created on and that device's scratch array (g_scratch_dev)
+	int device = -1;
+	void* scratchDev = nullptr;
 	// HIP graphs captured from this engine hold raw pointers to its buffers and cached tables: while
 	// any is alive (liveGraphs > 0) no buffer may grow (be reallocated) and no cached table may be
 	// dropped; the engine's destruction invalidates the graphs still alive (graphs)
@@ -205,7 +243,9 @@ struct DevBuf {
 	{
 		if (n <= bytes) return 0;
 		if (tl_capturing) return set_err(-1, "buffer growth during graph capture");
-
+		if (tl_noalloc)
+			return set_err(-1, "qbf_simulate_periodic_noalloc: an engine buffer would have to grow to %zu bytes; "
+				"run qbf_inhom_step_periodic once on the same cell count first", n);
 		if (lock && *lock)
 			return set_err(-1, "buffer growth (%zu bytes) while a captured graph uses the engine's buffers: "
 				"destroy the graph first, or capture after a call of the largest size", n);
@@ -683,7 +723,9 @@ struct EngineT final : public qbf_engine {
 				return 0;
 			}
 		if (tl_capturing) return set_err(-1, "combination table not prepared: run the step once before capturing it");
-
+		if (tl_noalloc)
+			return set_err(-1, "qbf_simulate_periodic_noalloc: combination table not on the device; run "
+				"qbf_inhom_step_periodic once on the same cell count first");
 		if (termCache.size() >= TERM_CACHE && liveGraphs)
 			return set_err(-1, "combination-table cache full while a captured graph uses its tables: destroy the graph first");
 		if (termCache.size() >= TERM_CACHE) {
@@ -1124,6 +1166,12 @@ int qbf_engine_create(int N, int precision, int maxCells, qbf_engine** out)
 	}
 	e->stream = s;
 	e->ownStream = true;
+	// the scratch array of qbf_simulate_periodic_noalloc on this device (looked up once per device)
+	r = scratch_for_current_device(&e->device, &e->scratchDev);
+	if (r) {
+		qbf_engine_destroy(e);
+		return r;
+	}
 	*out = e;
 	return 0;
 }
@@ -1516,6 +1564,29 @@ int qbf_simulate_maxwell(qbf_engine* e, const double* dW0, int ncell, double L, 
 	bc.WL = dWMaxwL;
 	bc.WR = dWMaxwR;
 	return simulate_alloc(e, dW0, ncell, L, h, dt, numsteps, stride, dBext, hostFrames, bc, "qbf_simulate_maxwell");
+}
+
+int qbf_simulate_periodic_noalloc(qbf_engine* e, const double* dW0, int ncell, double L, double h, double dt,
+	unsigned numsteps, unsigned stride, const double* dBext, double* hostFrames)
+{
+	if (!e || !dW0 || !hostFrames || ncell < 1 || stride < 1)
+		return set_err(-1, "qbf_simulate_periodic_noalloc: bad argument");
+	const size_t bytes = sizeof(double) * 4 * (size_t)e->gridN * e->gridN * ncell;
+	if (bytes > QBF_NOALLOC_STATE_BYTES)
+		return set_err(-1, "qbf_simulate_periodic_noalloc: a state of %d cells at N = %d is %zu bytes, above the "
+			"limit of %zu (QBF_NOALLOC_STATE_BYTES)", ncell, e->gridN, bytes, (size_t)QBF_NOALLOC_STATE_BYTES);
+	if (!e->scratchDev) return set_err(-2, "qbf_simulate_periodic_noalloc: no scratch address for the engine's device");
+	int dev = -1;
+	if (hipGetDevice(&dev) != hipSuccess || dev != e->device)
+		return set_err(-1, "qbf_simulate_periodic_noalloc: the current device (%d) is not the engine's (%d)", dev,
+			e->device);
+	// the two state buffers: both halves of the device's scratch array (every element is written
+	// by a step before it is read, so nothing needs clearing)
+	double* const buf[2] = {(double*)e->scratchDev, (double*)((char*)e->scratchDev + QBF_NOALLOC_STATE_BYTES)};
+	tl_noalloc = true;
+	const int r = simulate_loop(e, dW0, ncell, L, h, dt, numsteps, stride, dBext, hostFrames, buf, bytes);
+	tl_noalloc = false;
+	return r;
 }
 
 int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, double* dOut)

@@ -22,6 +22,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("QBF_LIB_PATH") or os.path.join(PKG_DIR, "lib", "libqbf.so")
 
 FP64, FP32, BF16X3 = 0, 1, 2
+NOALLOC_STATE_BYTES = 32 << 20      # QBF_NOALLOC_STATE_BYTES of include/qbf.h
 CC, CD, CFULL = 1, 2, 3
 TAB_I1, TAB_I4 = 0, 1
 CONS_MASS, CONS_MOMENTUM, CONS_ENERGY = 1, 2, 4
@@ -94,6 +95,7 @@ def lib():
             "qbf_fd_distance": (i, [vp, vp, i, d, vp, i, vp]),
             "qbf_cell_average": (i, [vp, vp, i, vp]),
             "qbf_simulate_periodic": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
+            "qbf_simulate_periodic_noalloc": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
             "qbf_simulate_dirichlet": (i, [vp, vp, i, d, d, d, u, u, vp, vp]),
             "qbf_simulate_maxwell": (i, [vp, vp, i, d, d, d, d, vp, vp, u, u, vp, vp]),
             "qbf_profile": (i, [vp, i]),
@@ -229,14 +231,17 @@ class Engine:
         _check(lib().qbf_moment_correct(self._h, self._R(Wold, "Wold"), self._R(Wnew, "Wnew", n), n, L, mask))
         return Wnew
 
-    def simulate_periodic(self, W0, L, h, dt, numsteps, stride=1, Bext=None):
+    def simulate_periodic(self, W0, L, h, dt, numsteps, stride=1, Bext=None, noalloc=False):
         """Frames 0, stride, 2 stride, ... (< numsteps) of the periodic problem from W0, as a
-        numpy array [nframes, ncell, 4, N, N] (the state stays on the device between steps)."""
+        numpy array [nframes, ncell, 4, N, N] (the state stays on the device between steps).
+        noalloc: the allocation-free variant (qbf_simulate_periodic_noalloc: states of at most
+        NOALLOC_STATE_BYTES, engine buffers already sized by a step on as many cells)."""
         self._bind()
         nf = (numsteps - 1) // stride + 1 if numsteps > 0 else 0
         out = np.empty((max(nf, 1),) + tuple(W0.shape))
-        _check(lib().qbf_simulate_periodic(self._h, self._R(W0), W0.shape[0], L, h, dt, numsteps, stride,
-                                           self._B(Bext, W0.shape[0]), out.ctypes.data_as(C.c_void_p)))
+        fn = lib().qbf_simulate_periodic_noalloc if noalloc else lib().qbf_simulate_periodic
+        _check(fn(self._h, self._R(W0), W0.shape[0], L, h, dt, numsteps, stride, self._B(Bext, W0.shape[0]),
+                  out.ctypes.data_as(C.c_void_p)))
         return out[:nf]
 
     def simulate_dirichlet(self, W0, L, h, dt, numsteps, stride=1, Bext=None):
