@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--paths", default="surrogate,mixed,bf16x3,fp64")
     ap.add_argument("--mixed-steps", type=int, default=3, help="timed steps of the FP32 exact path")
     ap.add_argument("--fp64-steps", type=int, default=1, help="timed FP64 plain steps")
+    ap.add_argument("--compare-steps", type=int, default=0,
+                    help="without a stored FP64 plain trajectory of the configuration: run one of this many steps "
+                         "first and check the other paths against it at its last step")
     ap.add_argument("--c3-steps", type=int, default=200,
                     help="continue the FP32 and BF16-split exact trajectories (untimed) to this step for their "
                          "accuracy check against the FP64 plain trajectory (0: no check beyond the timed steps)")
@@ -232,6 +235,26 @@ def main():
                 "kept_cells": [int(c) for c in ref["cells"]]}
 
     out = {}
+    if args.compare_steps > 0 and ref is None and "fp64" in paths:
+        # no stored FP64 plain trajectory for this configuration: run one here first (its steps after
+        # the first are the FP64 timing) and keep its moments and the state of 16 cells at the end
+        e64 = make(qbf.FP64)
+        step64 = lambda a, b: e64.inhom_step_periodic(a, wl.L_BOX, h, dt, out=b)  # noqa: E731
+        W64_1, _ = run_steps(step64, W0, 1)
+        note_footprint(qbf.FP64)
+        t0 = time.perf_counter()
+        WK, _ = run_steps(step64, W64_1, args.compare_steps - 1)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        nk = args.compare_steps - 1
+        out["fp64"] = {"value": evals_per_step * nk / t64, "ms_per_step": 1e3 * t64 / nk, "steps": nk, "W1": W64_1}
+        keep = np.linspace(0, Nx - 1, 16).astype(np.int64)
+        ref = {"moments": np.stack([wl.moments(w0), wl.moments(WK.cpu().numpy())]), "every": args.compare_steps,
+               "steps": args.compare_steps, "cells": keep, "final": WK[torch.from_numpy(keep).to(dev)].cpu().numpy(),
+               "source": f"FP64 plain trajectory of {args.compare_steps} steps run in this bench"}
+        progress(f"FP64 plain reference: {args.compare_steps} steps, {1e3 * t64 / nk:.0f} ms per step")
+        del e64, WK
+        torch.cuda.empty_cache()
     # the accuracy check: the step of the reference's last frame (200), reached by the fast path's own
     # trajectory (the timed steps, continued untimed when there are fewer)
     check_step = int(ref["steps"]) if ref is not None else None
@@ -290,7 +313,7 @@ def main():
         del eb
 
     # ---------------------------------------------------------------- FP64 plain path
-    if "fp64" in paths:
+    if "fp64" in paths and "fp64" not in out:
         e64 = make(qbf.FP64)
         step64 = lambda a, b: e64.inhom_step_periodic(a, wl.L_BOX, h, dt, out=b)  # noqa: E731
         W64_1, _ = run_steps(step64, W0, 1)
@@ -343,6 +366,9 @@ def main():
                                       sample_frac=args.sample_frac, tol=args.tol, mask=qbf.CONS_ALL, seed=args.seed)
         warm = make_stepper()
         run_steps(lambda a, b: warm.step(a, out=b), W0, args.warmup)
+        if qbf.FP32 not in footprint:
+            torch.cuda.empty_cache()        # the training data (freed) out of torch's cache
+            note_footprint(qbf.FP32)        # the exact engine and the network with its buffers
         st = make_stepper()
         torch.cuda.synchronize()
         e32.profile(True)
@@ -465,7 +491,9 @@ def main():
                           "criterion": "after 200 steps against the FP64 plain trajectory: density and energy per cell "
                                        "<= 1e-3 (relative to the largest cell value) and the whole state of the kept "
                                        "cells <= 1e-3 (L1 relative)",
-                          "passed": passes(F["error_vs_fp64"]) if fast == "surrogate" else None},
+                          "passed": passes(F["error_vs_fp64"]) if fast == "surrogate" else None,
+                          "reference": (ref["source"] if isinstance(ref, dict) else
+                                        "tests/golden/periodic64_fp64_ref.npz" if ref is not None else None)},
     }
     # host-buffer boundary (the drop-in entry points): the state goes in and comes back once per
     # step over PCIe; pinned buffers, copies timed on their own, outside the timed region

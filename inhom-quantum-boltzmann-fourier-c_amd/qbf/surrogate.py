@@ -112,6 +112,8 @@ def trajectory_set(engine, seeds, t_end=0.04, Nx=32, per_seed=None, rng_seed=0, 
             xs, ys = xs.index_select(0, idx), ys.index_select(0, idx)
         X.append(xs)
         Y.append(ys)
+        if len(X) % 8 == 0:
+            print(f"  training trajectories: {len(X)}/{len(seeds)} seeds", file=sys.stderr, flush=True)
     return torch.cat(X).contiguous(), torch.cat(Y).contiguous()
 
 
