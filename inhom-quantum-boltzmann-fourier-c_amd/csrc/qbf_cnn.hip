@@ -220,15 +220,17 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 		}
 }
 
-// Weight and bias gradients on the VALU:
-//   dW[t][co][ci] += sum_p dz[p][co] * x[p + dil off_t][ci],  db[co] += sum_p dz[p][co]
-// One workgroup per (image, block of ROWS rows); thread (co, 16-channel ci block) keeps
-// 9 x 16 accumulators; the three rows of x that a row of dz meets and that row of dz are
-// staged in LDS (periodic in x; the x rows with a pitch of CI + 4 floats, so that a thread reads
-// its 16 channels as four aligned 16-byte words).
+// Weight and bias gradients on the VALU, per workgroup:
+//   part[blk][(t * CO + co) * CI + ci] = sum_p dz[p][co] * x[p + dil off_t][ci],
+//   part[blk][9 CO CI + co] = sum_p dz[p][co]   (p over the workgroup's rows)
+// summed over the workgroups in a fixed order by k_wgrad_reduce (no atomics: the gradients are the
+// same bit for bit from run to run).  One workgroup per (image, block of ROWS rows); thread (co,
+// 16-channel ci block) keeps 9 x 16 accumulators; the three rows of x that a row of dz meets and that
+// row of dz are staged in LDS (periodic in x; the x rows with a pitch of CI + 4 floats, so that a
+// thread reads its 16 channels as four aligned 16-byte words).
 template <int S, int CI, int CO>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dz,
-	float* __restrict__ dW, float* __restrict__ db, int rows, int dil)
+	float* __restrict__ part, int rows, int dil)
 {
 	constexpr int NCB = CI / 16;                 // ci blocks of 16
 	constexpr int NTH = CO * NCB;                // active threads
@@ -273,19 +275,34 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ x, 
 		}
 	}
 	if (tid < NTH) {
+		float* pb = part + (size_t)blockIdx.x * (9 * CO * CI + CO);
 #pragma unroll
 		for (int t = 0; t < 9; t++)
 #pragma unroll
-			for (int j = 0; j < 16; j++) atomicAdd(&dW[((size_t)t * CO + co) * CI + cb * 16 + j], acc[t][j]);
-		if (cb == 0) atomicAdd(&db[co], bacc);
+			for (int j = 0; j < 16; j++) pb[((size_t)t * CO + co) * CI + cb * 16 + j] = acc[t][j];
+		if (cb == 0) pb[9 * CO * CI + co] = bacc;
 	}
 }
 
-// loss = mean over elements of d^2, gradient with respect to the network output -> bf16 NHWC
-// (CIO channels).  d = pred - target, or (pred - target) / (oscale a[b]^2) with amp.
+// dW[i] = sum_blk part[blk][i] (i < nw), db[i - nw] likewise (nw <= i < nw + nb), blocks in order
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int nblk, int nw, int nb, float* __restrict__ dW,
+	float* __restrict__ db)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int stride = nw + nb;
+	if (i >= stride) return;
+	float s = 0.f;
+	for (int b = 0; b < nblk; b++) s += part[(size_t)b * stride + i];
+	if (i < nw) dW[i] = s;
+	else db[i - nw] = s;
+}
+
+// loss = mean over elements of d^2 (per-workgroup partials into lpart, summed by k_loss_reduce),
+// gradient with respect to the network output -> bf16 NHWC (CIO channels).  d = pred - target, or
+// (pred - target) / (oscale a[b]^2) with amp.
 template <int S>
-__global__ void k_loss_grad(const float* __restrict__ pred, const float* __restrict__ target, bf16* __restrict__ g,
-	float* __restrict__ loss, int B, float inv_n, float oscale, const float* __restrict__ amp)
+__global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ pred, const float* __restrict__ target,
+	bf16* __restrict__ g, float* __restrict__ lpart, int B, float inv_n, float oscale, const float* __restrict__ amp)
 {
 	const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
 	const long long n = (long long)B * S * S;
@@ -306,7 +323,25 @@ __global__ void k_loss_grad(const float* __restrict__ pred, const float* __restr
 		}
 	}
 	for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-	if ((threadIdx.x & 63) == 0) atomicAdd(loss, part * inv_n);
+	__shared__ float wsum[4];
+	if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
+	__syncthreads();
+	if (threadIdx.x == 0) lpart[blockIdx.x] = ((wsum[0] + wsum[1]) + (wsum[2] + wsum[3])) * inv_n;
+}
+
+// loss = sum of the per-block partials, in order (one workgroup of 256)
+__global__ __launch_bounds__(256) void k_loss_reduce(const float* __restrict__ lpart, int n, float* __restrict__ loss)
+{
+	__shared__ float sh[256];
+	float s = 0.f;
+	for (int i = threadIdx.x; i < n; i += 256) s += lpart[i];
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) *loss = sh[0];
 }
 
 // Adam on fp32 master weights, then bf16 copies W[t][co][ci] and W^T[t][ci][co].
@@ -465,6 +500,10 @@ struct qbf_cnn {
 	float* z[NL] = {};
 	bf16* g[NL] = {};     // gradient of z[l] (bf16 NHWC), g[NL-1] = gradient of the output
 	float* loss = nullptr;
+	// deterministic reductions: per-workgroup partials of the weight gradients of one layer and of the loss
+	static constexpr int WG_ROWS = 16;    // image rows per workgroup of k_conv_wgrad
+	float* gpart = nullptr;
+	float* lpart = nullptr;
 	int dil[NL] = {1, 1, 1, 1, 1, 1};
 	int norm = 0;                         // per-cell normalisation
 	float* amp = nullptr;                 // per-cell amplitude [maxB]
@@ -534,6 +573,8 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		}
 	}
 	c->loss = c->alloc_f(1);
+	c->gpart = c->alloc_f((size_t)c->maxTrain * (S / qbf_cnn::WG_ROWS) * (9 * CH * CH + CH));
+	c->lpart = c->alloc_f(pxt / 256 + 1);
 	c->amp = c->alloc_f(maxBatch);
 	c->sx = c->alloc_f(px * CREAL);
 	c->sc = c->alloc_f(px * CREAL);
@@ -559,7 +600,7 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		c->swc = (double*)p3;
 		c->sres = (double*)p4;
 	}
-	if (!c->loss || !c->amp || !c->sx || !c->sc || !c->soff || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
+	if (!c->loss || !c->gpart || !c->lpart || !c->amp || !c->sx || !c->sc || !c->soff || !c->sw || !c->sw2 || !c->sidx || !c->swf || !c->swc || !c->sres) {
 		delete c;
 		return cnn_err(-3, "qbf_cnn_create: out of device memory");
 	}
@@ -695,31 +736,36 @@ static int gradients_t(qbf_cnn* c, const float* dIn, const float* dTarget, int B
 {
 	int r = run_forward_t<S>(c, dIn, B, iscale, dOut, oscale, true);
 	if (r) return r;
-	for (int l = 0; l < NL; l++) {
-		HCHK(hipMemsetAsync(c->gw[l], 0, wsize(l) * sizeof(float), c->stream));
-		HCHK(hipMemsetAsync(c->gb[l], 0, cout_of(l) * sizeof(float), c->stream));
-	}
-	HCHK(hipMemsetAsync(c->loss, 0, sizeof(float), c->stream));
 	const long long npx = (long long)B * S * S;
 	const float inv_n = 1.0f / (float)(npx * CREAL);
-	hipLaunchKernelGGL(k_loss_grad<S>, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dOut, dTarget,
-		c->g[NL - 1], c->loss, B, inv_n, oscale, c->norm ? c->amp : nullptr);
+	const int nlb = (int)((npx + 255) / 256);
+	hipLaunchKernelGGL(k_loss_grad<S>, dim3((unsigned)nlb), dim3(256), 0, c->stream, dOut, dTarget, c->g[NL - 1],
+		c->lpart, B, inv_n, oscale, c->norm ? c->amp : nullptr);
+	hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, c->stream, c->lpart, nlb, c->loss);
 	const dim3 grid = conv_grid<S>(B), cblk(64 * CROWS), blk(256);
-	const int rows = 16;
-	const dim3 wgrid(B * (S / rows));
+	const int rows = qbf_cnn::WG_ROWS;
+	const int nwb = B * (S / rows);
+	const dim3 wgrid(nwb);
+	auto reduce = [&](int l) {
+		const int nw = (int)wsize(l), nb = cout_of(l);
+		hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((nw + nb + 255) / 256)), dim3(256), 0, c->stream, c->gpart,
+			nwb, nw, nb, c->gw[l], c->gb[l]);
+	};
 	// last layer: weight gradient, then data gradient into the last hidden pre-activation
-	hipLaunchKernelGGL((k_conv_wgrad<S, CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gw[NL - 1],
-		c->gb[NL - 1], rows, c->dil[NL - 1]);
+	hipLaunchKernelGGL((k_conv_wgrad<S, CH, CIO>), wgrid, blk, 0, c->stream, c->x[NL - 1], c->g[NL - 1], c->gpart, rows,
+		c->dil[NL - 1]);
+	reduce(NL - 1);
 	hipLaunchKernelGGL((k_conv_mfma<S, CIO, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[NL - 1], c->wbt[NL - 1], nullptr,
 		nullptr, c->g[NL - 2], nullptr, c->z[NL - 2], 1.0f, c->dil[NL - 1], nullptr);
 	for (int l = NL - 2; l >= 1; l--) {
-		hipLaunchKernelGGL((k_conv_wgrad<S, CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gw[l], c->gb[l], rows,
+		hipLaunchKernelGGL((k_conv_wgrad<S, CH, CH>), wgrid, blk, 0, c->stream, c->x[l], c->g[l], c->gpart, rows,
 			c->dil[l]);
+		reduce(l);
 		hipLaunchKernelGGL((k_conv_mfma<S, CH, CH, -1, 2>), grid, cblk, 0, c->stream, c->g[l], c->wbt[l], nullptr, nullptr,
 			c->g[l - 1], nullptr, c->z[l - 1], 1.0f, c->dil[l], nullptr);
 	}
-	hipLaunchKernelGGL((k_conv_wgrad<S, CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gw[0], c->gb[0], rows,
-		c->dil[0]);
+	hipLaunchKernelGGL((k_conv_wgrad<S, CIO, CH>), wgrid, blk, 0, c->stream, c->x[0], c->g[0], c->gpart, rows, c->dil[0]);
+	reduce(0);
 	HCHK(hipGetLastError());
 	if (hostLoss) {
 		HCHK(hipMemcpyAsync(hostLoss, c->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));

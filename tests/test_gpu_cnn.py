@@ -162,3 +162,27 @@ def test_training_reduces_loss():
     print("loss", losses[0], "->", losses[-1])
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.2 * losses[0]
+
+
+def test_training_is_reproducible():
+    """The weight and bias gradients and the loss are summed over the workgroups in a fixed order (no
+    atomics), so two trainings from the same weights on the same batches end bit for bit equal."""
+    import torch
+    from qbf.cnn import CNN
+    p, x, _ = _setup(5, B=8)
+    t = (0.5 * np.tanh(x) - 0.2 * np.roll(x, 1, axis=3)).astype(np.float32)
+    X, T = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    runs = []
+    for _ in range(2):
+        net = CNN(max_batch=8, params=R.init_params(5))
+        losses = []
+        for it in range(10):
+            loss, _ = net.gradients(X, T, iscale=1.1, oscale=0.7)
+            losses.append(loss)
+            net.adam(2e-3)
+        runs.append((net.get_params(), net.get_params(grads=True), losses))
+        net.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
+    assert not np.array_equal(runs[0][0], R.init_params(5))
