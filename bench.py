@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--sample-frac", type=float, default=0.02)
     ap.add_argument("--tol", type=float, default=1e-3,
                     help="fallback above this relative state error projected to the next exact step")
+    ap.add_argument("--graphs", action="store_true",
+                    help="surrogate policy: the exact and the surrogate Strang steps replayed from captured HIP "
+                         "graphs (the paired-path choice of the first eager exact step is then kept)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true", help="run a 256 MiB device copy first")
@@ -361,15 +364,16 @@ def main():
         torch.cuda.synchronize()
         cnn_ms = ev0.elapsed_time(ev1) / 5
 
-        def make_stepper():
+        def make_stepper(graphs=False):
             return S.SurrogateStepper(e32, net, isc, osc, wl.L_BOX, h, dt, exact_every=args.exact_every,
-                                      sample_frac=args.sample_frac, tol=args.tol, mask=qbf.CONS_ALL, seed=args.seed)
+                                      sample_frac=args.sample_frac, tol=args.tol, mask=qbf.CONS_ALL, seed=args.seed,
+                                      graphs=graphs)
         warm = make_stepper()
         run_steps(lambda a, b: warm.step(a, out=b), W0, args.warmup)
         if qbf.FP32 not in footprint:
             torch.cuda.empty_cache()        # the training data (freed) out of torch's cache
             note_footprint(qbf.FP32)        # the exact engine and the network with its buffers
-        st = make_stepper()
+        st = make_stepper(args.graphs)
         torch.cuda.synchronize()
         e32.profile(True)
         t0 = time.perf_counter()
@@ -391,6 +395,10 @@ def main():
             err = moment_errors(Wc, check_step)
             err["policy_counts_to_check"] = dict(st.counts)
             del Wc
+        graphs_info = ({"captured": len(st._graphs), "replays": st.replays,
+                        "note": "steps replayed from captured HIP graphs; the kernel profile counts the eager "
+                                "launches only"} if args.graphs else None)
+        st.close()
         # the network-free fillers at the same cadence (exact step every exact_every, conservation
         # correction), each timed over its own trajectory to the check step
         fillers = {}
@@ -414,6 +422,7 @@ def main():
             "residual_median": float(np.median(st.residuals)) if st.residuals else None,
             "residual_max": float(np.max(st.residuals)) if st.residuals else None,
             "operator_residual_median": float(np.median(st.op_residuals)) if st.op_residuals else None,
+            "graphs": graphs_info,
             "training": {"data": (f"exact FP32 trajectories (with conservation correction) of seeds 100-"
                                   f"{99 + args.train_seeds}, {args.train_nx} cells each, to t = 0.04: every step's "
                                   "(state, collision operator) pairs" if args.train_data == "trajectory" else
@@ -456,7 +465,8 @@ def main():
     achieved = kapps * flop_per_k / (kms / 1e3) if kms > 0 else 0.0
     headline = (N, Nx) == (64, 512)
     i3_kernel = ("k_i3acc<float,64,1024,false,%s> (I3 K-operator: IFFT2 -> psi2 -> FFT2)" % ("true" if used else "false")
-                 if N == 64 else f"k_i3acc_big<float> (I3 K-operator on {P} x {P} grids in global memory)" if N == 128
+                 if N == 64 else f"k_i3acc_big<float,{'true' if used else 'false'}> (I3 K-operator on {P} x {P} grids "
+                 "in global memory)" if N == 128
                  else f"k_i3acc<float,{N},...> (I3 K-operator)")
     line = {
         "metric": METRIC if headline else f"collision evals/s at N={N},Nx={Nx}: FP32 exact vs FP64 plain",

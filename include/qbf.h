@@ -48,7 +48,7 @@ typedef struct {
 const char *qbf_last_error(void);
 
 /* N = 16, 32, 64 (padded grids held in registers/LDS by one workgroup) or 128 (padded 256 x 256 grids
- * in global memory; no paired path).  The reference fixes N at compile time (include/grid.h:34). */
+ * in global memory).  The reference fixes N at compile time (include/grid.h:34). */
 int  qbf_engine_create(int N, int precision, int maxCells, qbf_engine **out);
 void qbf_engine_destroy(qbf_engine *e);
 int  qbf_engine_N(const qbf_engine *e);
@@ -189,7 +189,7 @@ int qbf_cell_diagnostics(qbf_engine *e, const double *dW, int ncell, double L, d
  * spectra of real states every grid that K_q = FFT2(psi2 .* IFFT2(.)) acts on is real, so one complex
  * application carries two, and the I1 and I2 chains of two components share one transform; the
  * imaginary parts this drops come from the Nyquist lines of the state (zero padding places them
- * without their conjugate partners).  mode 0: off; 1: on when the
+ * without their conjugate partners; at N = 128 the I2 chains stay unpaired).  mode 0: off; 1: on when the
  * measured max |Im| / max |Re| of the padded physical grids of the state is at most tol; 2: always.
  * The paired result deviates from the complex one by about 0.15-0.8 times the ratio.  Defaults:
  * FP64 engines 0, FP32 engines 1 with tol = 2e-5.  qbf_real_pairs_status: whether the last collision evaluation used it, and the ratio

@@ -274,7 +274,9 @@ __global__ __launch_bounds__(big::TB<R>) void k_i2acc_big(I2Args a)
 // output (at most 4) go through each pass together, in NI = 4 scratch grids per workgroup: rows
 // (IFFT along k of S .* H), columns (IFFT along l, times psi2_q read once, FFT along l), rows (FFT
 // along k, times w F, summed over the instances and added to the accumulator slot once per q).
-template <typename R>
+// PAIR: packed pairs of real grids, as k_i3acc with PAIR (H one component of a packed F grid,
+// instH = 2 slot + component; two weights per instance; the accumulator's real part only).
+template <typename R, bool PAIR>
 __global__ __launch_bounds__(big::TB<R>) void k_i3acc_big(I3Args a)
 {
 	using namespace big;
@@ -311,11 +313,23 @@ __global__ __launch_bounds__(big::TB<R>) void k_i3acc_big(I3Args a)
 			const int rb = (sw * L + lt) * P;
 #pragma unroll 1
 			for (int in = 0; in < ni; in++) {
+				const int hs = a.instH[out][in];
 				const cx<R>* Sg = uni(Sh, (long long)a.instS[out][in] * PP) + rb + r0;
-				const cx<R>* Fh = uni(Fq, (long long)a.instH[out][in] * PP) + rb + r0;
+				const cx<R>* Fh = uni(Fq, (long long)(PAIR ? hs >> 1 : hs) * PP) + rb + r0;
 				cx<R> v[E];
+				if constexpr (PAIR) {
+					const bool hy = hs & 1;
 #pragma unroll
-				for (int u = 0; u < E; u++) v[u] = cmul<R>(Sg[u], Fh[u]);
+					for (int u = 0; u < E; u++) {
+						const cx<R> hh = Fh[u];
+						const R hv = hy ? hh.y : hh.x;
+						const cx<R> sv = Sg[u];
+						v[u] = mk<R>(sv.x * hv, sv.y * hv);
+					}
+				} else {
+#pragma unroll
+					for (int u = 0; u < E; u++) v[u] = cmul<R>(Sg[u], Fh[u]);
+				}
 				line_bwd<R, P, G, E>(v, p, tw);
 				cx<R>* x = uni(xs, (long long)in * PP) + rb + p;
 #pragma unroll
@@ -361,12 +375,21 @@ __global__ __launch_bounds__(big::TB<R>) void k_i3acc_big(I3Args a)
 				for (int s = 0; s < E; s++) v[s] = x[G * s];
 				line_fwd<R, P, G, E>(v, p, tw);
 				const cx<R>* Ff = uni(Fq, (long long)a.instF[out][in] * PP) + rb + r0;
-				const R w = (R)a.instW[out][in];
+				if constexpr (PAIR) {
+					const R wa = (R)a.instW[out][2 * in], wb = (R)a.instW[out][2 * in + 1];
 #pragma unroll
-				for (int u = 0; u < E; u++) {
-					const cx<R> y = cmul<R>(Ff[u], v[u]);
-					sum[u].x += w * y.x;
-					sum[u].y += w * y.y;
+					for (int u = 0; u < E; u++) {
+						const cx<R> f = Ff[u];
+						sum[u].x += (wa * f.x) * v[u].x + (wb * f.y) * v[u].y;
+					}
+				} else {
+					const R w = (R)a.instW[out][in];
+#pragma unroll
+					for (int u = 0; u < E; u++) {
+						const cx<R> y = cmul<R>(Ff[u], v[u]);
+						sum[u].x += w * y.x;
+						sum[u].y += w * y.y;
+					}
 				}
 			}
 			cx<R>* row = accp + rb + r0;

@@ -117,6 +117,11 @@ struct qbf_graph {
 	hipStream_t stream = nullptr;    // the engine's stream at capture time; replays launch there
 	struct qbf_engine* e = nullptr;         // null once the engine is destroyed (the graph is then dead)
 	hipEvent_t before = nullptr, after = nullptr;   // order replays against the engine's current stream
+	// the engine's record of its last periodic step (qbf_last_collision) as the captured calls left
+	// it; a replay sets it again (setsLast) when the calls changed it
+	bool setsLast = false;
+	int lastCells = 0;
+	double lastDt = 0.0;
 	void release()
 	{
 		if (exec) (void)hipGraphExecDestroy(exec);
@@ -192,6 +197,7 @@ struct qbf_engine {
 	// the last periodic Strang step: its cell count and dt (0 cells: none since the last other step)
 	int lastCells = 0;
 	double lastDt = 0.0;
+	unsigned lastSerial = 0;    // counts the updates of lastCells / lastDt (graph capture compares it)
 	virtual int last_collision(int ncell, double* Win, double* C) = 0;
 	virtual int inhom_step_dirichlet(const double* W, int ncell, double L, double h, double dt, const double* Bext,
 		double* out) = 0;
@@ -506,7 +512,7 @@ struct EngineT final : public qbf_engine {
 		int* nchunk, bool pair = false)
 	{
 		if (n2 <= 0) return set_err(-1, "I2 table not set");
-		if (BIG && pair) return set_err(-1, "no paired path at N = 128");
+		if (BIG && pair) return set_err(-1, "no paired I2 kernel at N = 128");
 		if (pair)
 			for (int i = 0; i < ninst; i++)
 				if (instF[i] != instG[i]) return set_err(-1, "paired I2 needs f = g");
@@ -539,7 +545,6 @@ struct EngineT final : public qbf_engine {
 		int ncell, int* nchunkOut, bool pair = false)
 	{
 		if (n3 <= 0) return set_err(-1, "I3 table not set");
-		if (BIG && pair) return set_err(-1, "no paired path at N = 128");
 		const int nq = n3;
 		// term ranges: about 1024 workgroups when there are few cells, a power of two so that the
 		// ranges of a power-of-two term count are equal and the workgroups fill whole waves
@@ -575,9 +580,12 @@ struct EngineT final : public qbf_engine {
 			if (r) return r;
 			hipEvent_t eb = nullptr, ee = nullptr;
 			if (prof && !tl_capturing && prof_begin(&eb, &ee) == 0) HIPCHK(hipEventRecord(eb, stream));
-			if constexpr (BIG)
-				hipLaunchKernelGGL((k_i3acc_big<R>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
-			else if (mfmaI3) {
+			if constexpr (BIG) {
+				if (pair)
+					hipLaunchKernelGGL((k_i3acc_big<R, true>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+				else
+					hipLaunchKernelGGL((k_i3acc_big<R, false>), dim3(ncell * nout * nchunk), dim3(T), 0, stream, a);
+			} else if (mfmaI3) {
 				if constexpr (std::is_same<R, float>::value && N == 64 && T == 1024) {
 					a.psi2n = i3p2m.p;
 					if (pair)
@@ -782,13 +790,13 @@ struct EngineT final : public qbf_engine {
 		// chain is complex-linear in its source, so W_a + i W_b yields out_a + i out_b
 		pairUsed = 0;
 		pairRatio = -1.0;
-		if (!BIG && pairMode == 1 && tl_capturing)
+		if (pairMode == 1 && tl_capturing)
 			return set_err(-1, "graph capture needs a fixed paired-path choice (qbf_set_real_pairs mode 0 or 2)");
-		if (!BIG && pairMode == 1) {
+		if (pairMode == 1) {
 			r = measure_realness(ncell, &pairRatio);
 			if (r) return r;
 		}
-		const bool pair = !BIG && (pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol));
+		const bool pair = pairMode == 2 || (pairMode == 1 && pairRatio <= pairTol);
 		if (pair) {
 			pairUsed = 1;
 			r = pack_sources(W, ncell);
@@ -820,7 +828,8 @@ struct EngineT final : public qbf_engine {
 			} else {
 				r = launch_i1(QBF_TAB_I4, W, 4, What.p, 4, 2.0, -1.0, 4, all4, zero4, Bhat, ncell, &nchB);
 			}
-			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf, ncell, &nchS, pair);
+			// (no paired I2 kernel at N = 128: its I2 runs on the complex sources)
+			if (!r) r = launch_i2(W, 4, 4, all4, all4, Sbuf, ncell, &nchS, pair && !BIG);
 			if (r) return r;
 			if (pair) {
 				r = run_i3_pairs(ncell, &nchunk);
@@ -888,6 +897,7 @@ struct EngineT final : public qbf_engine {
 	int collision_step(const double2* W, int ncell, double dt, const double* Bext, double2* Wnext) override
 	{
 		lastCells = 0;      // Wmid is rewritten (inhom_step sets it again afterwards)
+		lastSerial++;
 		int r = Wmid.ensure(sizeof(double2) * 4 * NN * (size_t)ncell);
 		if (r) return r;
 		double2* Wm = Wmid.template as<double2>();
@@ -1033,6 +1043,7 @@ struct EngineT final : public qbf_engine {
 		if (!r) r = transport(Wt2.template as<double>(), ncell, L, h, 0.5 * dt, out);
 		lastCells = r ? 0 : ncell;
 		lastDt = dt;
+		lastSerial++;
 		return r;
 	}
 
@@ -1121,7 +1132,7 @@ int make_engine(int maxCells, qbf_engine** out, int precision)
 		e->precision = QBF_BF16X3;
 	}
 	e->maxCells = maxCells;
-	if (std::is_same<R, float>::value && N != 128) {
+	if (std::is_same<R, float>::value) {
 		// the paired path deviates from the complex one by 0.15 (Nyquist content of smooth states) to
 		// 0.8 (grid-scale noise) times the measured ratio; below 2e-5 that stays under the FP32 path's
 		// own deviation from FP64 (2.1e-5 L1 at N = 64, J = 32, M = 16)
@@ -1352,10 +1363,12 @@ int qbf_graph_capture_calls(qbf_engine* e, int (*fn)(void*), void* ctx, qbf_grap
 	const bool prof = e->prof;
 	e->prof = false;
 	HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+	const unsigned serialBefore = e->lastSerial;
 	tl_capturing = true;
 	r = fn(ctx);
 	tl_capturing = false;
 	e->prof = prof;
+	const bool setsLast = e->lastSerial != serialBefore;
 	hipGraph_t graph = nullptr;
 	const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
 	if (r || ce != hipSuccess) {
@@ -1366,6 +1379,9 @@ int qbf_graph_capture_calls(qbf_engine* e, int (*fn)(void*), void* ctx, qbf_grap
 	ng->graph = graph;
 	ng->stream = e->stream;
 	ng->e = e;
+	ng->setsLast = setsLast;
+	ng->lastCells = e->lastCells;
+	ng->lastDt = e->lastDt;
 	const hipError_t ie = hipGraphInstantiate(&ng->exec, graph, nullptr, nullptr, 0);
 	if (ie != hipSuccess) {
 		ng->release();
@@ -1403,7 +1419,7 @@ int qbf_graph_capture_inhom_step_periodic(qbf_engine* e, const double* W, int nc
 	const double* Bext, double* out, qbf_graph** g)
 {
 	if (!e || !W || !out || !g) return set_err(-1, "qbf_graph_capture_inhom_step_periodic: null argument");
-	if (e->pairMode == 1 && e->gridN != 128)
+	if (e->pairMode == 1)
 		return set_err(-1, "qbf_graph_capture_inhom_step_periodic: needs a fixed paired-path choice "
 			"(qbf_set_real_pairs mode 0 or 2)");
 	StepCall c{e, W, Bext, ncell, L, h, dt, out};
@@ -1426,6 +1442,10 @@ int qbf_graph_launch(qbf_graph* g)
 	if (other) {
 		HIPCHK(hipEventRecord(g->after, g->stream));
 		HIPCHK(hipStreamWaitEvent(cur, g->after, 0));
+	}
+	if (g->setsLast) {
+		g->e->lastCells = g->lastCells;
+		g->e->lastDt = g->lastDt;
 	}
 	return 0;
 }
@@ -1600,7 +1620,6 @@ int qbf_cell_diagnostics(qbf_engine* e, const double* dW, int ncell, double L, d
 int qbf_set_real_pairs(qbf_engine* e, int mode, double tol)
 {
 	if (!e || mode < 0 || mode > 2 || !(tol >= 0)) return set_err(-1, "qbf_set_real_pairs: bad argument");
-	if (e->gridN == 128 && mode != 0) return set_err(-1, "qbf_set_real_pairs: no paired path at N = 128");
 	e->pairMode = mode;
 	e->pairTol = tol;
 	return 0;

@@ -200,6 +200,7 @@ class Engine:
         _check(lib().qbf_engine_create(N, precision, max_cells, C.byref(h)))
         self._h = h
         self._torch = torch
+        self.pair_mode = 0 if precision == FP64 else 1     # the library's default (qbf_set_real_pairs)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -274,6 +275,7 @@ class Engine:
     def set_real_pairs(self, mode: int, tol: float = 0.0):
         """Paired I3 path: 0 off, 1 when the measured imaginary/real ratio <= tol, 2 always."""
         _check(lib().qbf_set_real_pairs(self._h, mode, tol))
+        self.pair_mode = mode
 
     def real_pairs_status(self):
         """(used, ratio) of the last collision evaluation."""

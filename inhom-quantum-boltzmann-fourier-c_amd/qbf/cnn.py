@@ -163,6 +163,15 @@ class CNN:
         self._offset = D
         _check(_lib().qbf_surrogate_set_offset(self._h, None if D is None else _ptr(D), 0 if D is None else D.shape[0]))
 
+    def update_offset(self, D):
+        """Set the offset field to the values of D: written in place into the current field when it
+        has D's shape (graphs captured with that field then see the new values), else set_offset(D)."""
+        cur = getattr(self, "_offset", None)
+        if cur is None or cur.shape != D.shape or cur.dtype != D.dtype:
+            self.set_offset(D)
+        else:
+            cur.copy_(D)
+
     def collision_step(self, engine, W, dt, iscale, oscale, L, mask=0, out=None):
         """Midpoint collision step of physical states W (float64 cuda [ncell, 4, N, N]) with the
         network as collision operator, followed by the moment correction selected by mask."""

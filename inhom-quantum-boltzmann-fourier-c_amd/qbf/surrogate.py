@@ -22,6 +22,14 @@ Stepping policy (SurrogateStepper.step), per Strang step of the periodic problem
     (Engine.last_collision: no extra evaluation); the network then supplies only the change of
     the operator since the exact step, and its systematic error cancels.
 
+Graph mode (`graphs=True`): the exact Strang step and the surrogate Strang step are captured as
+HIP graphs (Engine.capture_inhom_step_periodic, CNN.capture_inhom_step_periodic) the second time
+they run between the same input and output buffers, and replayed from then on; the residual
+checks and the anchoring stay eager between the replays, and the offset field is rewritten in
+place.  A captured step needs a fixed paired-path choice: an engine in the automatic mode is set,
+before the first capture, to the choice its last eager exact step made (Engine.set_real_pairs mode
+0 or 2).  The engine's buffers cannot grow while the stepper's graphs live (close() destroys them).
+
 Network-free fillers (`filler`), the baselines that show what the network gains: between the
 exact steps the collision is either dropped ("none": transport only) or frozen at the last exact
 operator ("frozen": C(W) ~ C(W0), with the same conservation correction); neither runs a residual
@@ -168,7 +176,7 @@ class SurrogateStepper:
     FILLERS = ("network", "frozen", "none")
 
     def __init__(self, engine, net, iscale, oscale, L, h, dt, exact_every=10, sample_frac=0.02, tol=1e-3,
-                 mask=CONS_ALL, check=True, seed=0, anchor=True, filler="network"):
+                 mask=CONS_ALL, check=True, seed=0, anchor=True, filler="network", graphs=False):
         if filler not in self.FILLERS:
             raise QBFError(f"filler must be one of {self.FILLERS}")
         if filler != "network" and not exact_every:
@@ -187,16 +195,56 @@ class SurrogateStepper:
         self.counts = {"exact": 0, "surrogate": 0, "fallback": 0}
         self.residuals = []          # r as defined above
         self.op_residuals = []       # sum|C_net - C| / sum|C|
+        self.graphs = graphs and filler == "network"
+        self._graphs = {}            # (kind, input address, output address, cells) -> Graph
+        self._seen = set()           # the same keys, run eagerly once
+        self.replays = 0
+        self._exact_pairs = False    # the paired-path choice of the last eager exact step
+
+    def close(self):
+        """Destroy the captured graphs (the engine's buffers can grow again)."""
+        for g in self._graphs.values():
+            g.close()
+        self._graphs.clear()
+        self._seen.clear()
+
+    def _run(self, kind, W, out, eager, capture):
+        """One step of the given kind from W into out: eagerly, or (graph mode) captured the second time
+        these buffers meet and replayed afterwards."""
+        if not self.graphs:
+            return eager(out)
+        if out is None:
+            out = W.new_empty(W.shape)
+        key = (kind, W.data_ptr(), out.data_ptr(), W.shape[0])
+        g = self._graphs.get(key)
+        if g is not None:
+            g.launch()
+            self.replays += 1
+        elif key in self._seen:
+            if kind == "exact" and self.e.pair_mode == 1:
+                self.e.set_real_pairs(2 if self._exact_pairs else 0)
+            self._graphs[key], _ = capture(out)       # runs the step once, then captures it
+        else:
+            eager(out)
+            self._seen.add(key)
+            if kind == "exact":
+                self._exact_pairs = self.e.real_pairs_status()[0]
+        return out
 
     def _exact(self, W, out):
         self.e.set_moment_correction(self.mask)
-        out = self.e.inhom_step_periodic(W, self.L, self.h, self.dt, out=out)
+        out = self._run("exact", W, out, lambda o: self.e.inhom_step_periodic(W, self.L, self.h, self.dt, out=o),
+                        lambda o: self.e.capture_inhom_step_periodic(W, self.L, self.h, self.dt, out=o))
         if self.filler == "frozen":
             self.C0 = self.e.last_collision(W.shape[0], with_state=False)[1]
         if self.anchor:
             W0, C0 = self.e.last_collision(W.shape[0])
             N0 = self.net.forward(W0.float().contiguous(), self.iscale, self.oscale)
-            self.net.set_offset((C0 - N0.double()).float().contiguous())
+            D = (C0 - N0.double()).float().contiguous()
+            if self.graphs:
+                self.net.update_offset(D)     # in place: the captured surrogate steps read it
+            else:
+                self.net.set_offset(D)
         return out
 
     def step(self, W, out=None):
@@ -216,8 +264,9 @@ class SurrogateStepper:
             if not (math.isfinite(r) and r <= self.tol):
                 kind = "fallback"
         if kind == "surrogate" and self.filler == "network":
-            out = self.net.inhom_step_periodic(self.e, W, self.L, self.h, self.dt, self.iscale, self.oscale,
-                                               self.mask, out=out)
+            args = (self.e, W, self.L, self.h, self.dt, self.iscale, self.oscale, self.mask)
+            out = self._run("surrogate", W, out, lambda o: self.net.inhom_step_periodic(*args, out=o),
+                            lambda o: self.net.capture_inhom_step_periodic(*args, out=o))
         elif kind == "surrogate":
             B = self.e.transport_periodic(W, self.L, self.h, 0.5 * self.dt)
             if self.filler == "frozen":

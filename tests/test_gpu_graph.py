@@ -13,7 +13,7 @@ torch = pytest.importorskip("torch")
 import qbf  # noqa: E402
 from qbf import workloads as wl  # noqa: E402
 
-CASES = [(32, qbf.FP64, 0), (64, qbf.FP32, 2), (64, qbf.FP32, 0), (128, qbf.FP32, 0)]
+CASES = [(32, qbf.FP64, 0), (64, qbf.FP32, 2), (64, qbf.FP32, 0), (128, qbf.FP32, 0), (128, qbf.FP32, 2)]
 
 
 @pytest.mark.parametrize("N,prec,pairs", CASES)
@@ -24,8 +24,7 @@ def test_graph_replays_equal_eager_steps(N, prec, pairs):
     e = qbf.Engine(N, prec, max_cells=Nx)
     e.set_quadrature(*qbf.quadrature(N, 4, L, wl.R_BALL, 2))
     e.set_moment_correction(qbf.CONS_ALL)
-    if N != 128:
-        e.set_real_pairs(pairs, 0.0)
+    e.set_real_pairs(pairs, 0.0)
     rng = np.random.default_rng(N + prec)
     B = torch.from_numpy(0.3 * rng.standard_normal((Nx, 3))).cuda()
     states = [torch.from_numpy(wl.periodic(N, Nx, seed=s)).cuda() for s in range(4)]
@@ -144,5 +143,50 @@ def test_surrogate_step_graph_replays_equal_eager_steps(N):
         assert torch.equal(out, ref), k
         assert not torch.equal(out, states[k])
     g.close()
+    net.close()
+    e.close()
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_policy_in_graph_mode_equals_the_eager_policy(N):
+    """SurrogateStepper(graphs=True): the exact and the surrogate Strang steps are captured the second
+    time they run between the same buffers and replayed afterwards, with the residual checks and the
+    anchoring eager in between.  Trajectory, decisions and residuals equal the eager policy's bit for
+    bit; the anchor after an exact-step replay reads that replay through last_collision, and the
+    surrogate replays read the offset field rewritten in place."""
+    from qbf import surrogate as S
+    from qbf.cnn import CNN
+    Nx = 6
+    L, h, dt = wl.L_BOX, 1.0 / Nx, wl.cfl_dt(N, Nx)
+    print(f"policy graph mode N={N}", flush=True)
+    e = qbf.Engine(N, qbf.FP32, max_cells=Nx)
+    e.set_quadrature(*qbf.quadrature(N, 4, L, wl.R_BALL, 2))
+    e.set_real_pairs(2, 0.0)
+    net = CNN(max_batch=Nx, seed=5, N=N)
+    W0 = torch.from_numpy(wl.periodic(N, Nx, seed=2)).cuda()
+    isc, osc = 2.0, 0.05
+
+    def run(graphs):
+        st = S.SurrogateStepper(e, net, isc, osc, L, h, dt, exact_every=3, sample_frac=0.34, tol=1e9, seed=1,
+                                graphs=graphs)
+        bufs = [W0.clone(), torch.empty_like(W0)]
+        traj = []
+        for i in range(14):
+            st.step(bufs[i % 2], out=bufs[1 - i % 2])
+            traj.append(bufs[1 - i % 2].clone())
+        torch.cuda.synchronize()
+        info = (dict(st.counts), list(st.residuals), st.replays, len(st._graphs))
+        st.close()
+        return traj, info
+
+    ref, (c0, r0, _, _) = run(False)
+    got, (c1, r1, replays, captured) = run(True)
+    # exact steps 0, 3, 6, 9, 12 and surrogate steps between, alternating buffers: each kind meets both
+    # buffer orders, is captured on its second run in an order and replayed on the third
+    assert captured == 4 and replays == 6, (captured, replays)
+    assert c0 == c1 == {"exact": 5, "surrogate": 9, "fallback": 0}, (c0, c1)
+    assert r0 == r1
+    for k, (a, b) in enumerate(zip(ref, got)):
+        assert torch.equal(a, b), k
     net.close()
     e.close()

@@ -104,14 +104,71 @@ def test_n128_inhom_step_periodic_vs_oracle(e64):
     assert err < TOL64
 
 
-def test_n128_has_no_paired_path():
-    e = qbf.Engine(N, qbf.FP32, max_cells=1)
+def _nyquist_free(WF):
+    WF = WF.clone()
+    WF[..., N // 2, :] = 0
+    WF[..., :, N // 2] = 0
+    return WF
+
+
+def test_n128_paired_path_exact_on_nyquist_free_states_fp64():
+    """k_i3acc_big with PAIR (and the paired I1 / I4 chains): on spectra of real states with empty
+    Nyquist lines every grid K_q acts on is real, and the paired evaluation equals the complex one up
+    to rounding, and the oracle."""
+    from qbf import workloads as wl
+    q = O.Quad(N, 4, L, R, 2)
+    e = qbf.Engine(N, qbf.FP64, max_cells=2)
     try:
-        assert e.real_pairs_status()[0] == 0
-        with pytest.raises(RuntimeError):
-            e.set_real_pairs(1, 1e-5)
+        e.set_quadrature(q.I1, q.I2, q.I3, q.I4)
+        WF = _nyquist_free(e.phys_to_fourier(dev(wl.periodic(N, 2, seed=1, kmax=8))))
+        print("N=128 paired path, FP64, 2 cells", flush=True)
+        e.set_real_pairs(0)
+        c0 = e.collision(WF)
+        e.set_real_pairs(2)
+        c1 = e.collision(WF)
+        used, _ = e.real_pairs_status()
+        d = float((c1 - c0).abs().sum() / c0.abs().sum())
+        print(f"paired vs complex {d:.3e}", flush=True)
+        assert used and d < 1e-13
+        W0 = WF[0].cpu().numpy()
+        want = O.CcInt(q, W0) + O.CdInt(q, W0)
+        err = l1rel(host(c1)[0], want)
+        print(f"paired vs oracle {err:.3e}")
+        assert err < TOL64
     finally:
         e.close()
+
+
+def test_n128_paired_path_fp32_bounded_by_measured_ratio():
+    """FP32 at N = 128 on workload states (modes k <= 8): the paired path deviates from the complex one
+    by at most the dropped imaginary parts, bounded by the measured ratio, and the automatic mode
+    (the FP32 default, tolerance 2e-5) takes it when the ratio is below the tolerance."""
+    from qbf import workloads as wl
+    nx = 4
+    tabs = qbf.quadrature(N, 4, L, R, 2)
+    e32 = qbf.Engine(N, qbf.FP32, max_cells=nx)
+    e64 = qbf.Engine(N, qbf.FP64, max_cells=nx)
+    try:
+        e32.set_quadrature(*tabs)
+        e64.set_quadrature(*tabs)
+        WF = e64.phys_to_fourier(dev(wl.periodic(N, nx, seed=3, kmax=8)))
+        print("N=128 paired path, FP32, 4 cells", flush=True)
+        ref = e64.collision(WF)
+        e32.set_real_pairs(0)
+        cc = e32.collision(WF)
+        e32.set_real_pairs(1, 2e-5)
+        cp = e32.collision(WF)
+        used, ratio = e32.real_pairs_status()
+        l1 = lambda a, b: float((a - b).abs().sum() / b.abs().sum())  # noqa: E731
+        ec, ep, d = l1(cc, ref), l1(cp, ref), l1(cp, cc)
+        print(f"ratio {ratio:.3e} used {used}: complex vs FP64 {ec:.3e}, paired vs FP64 {ep:.3e}, "
+              f"paired vs complex {d:.3e}")
+        assert used == (ratio <= 2e-5)
+        assert d < 1e-5 + 10 * ratio
+        assert ep < 2 * ec + 10 * ratio
+    finally:
+        e32.close()
+        e64.close()
 
 
 def test_n128_boundary_steps_vs_oracle(e64):
