@@ -14,7 +14,7 @@ transport, over all Nx=512 cells on one GPU: 1024 collision evals per step.  Pat
 the fast path are timed from the initial state.  The accuracy check is made after 200 steps of the
 same trajectory (continued untimed when K < 200): the density and energy per cell against the FP64
 plain trajectory in tests/golden/periodic64_fp64_ref.npz (frames every 20 steps), and the whole
-state on the four cells whose FP64 state that file keeps.  The two network-free fillers (collision
+state on the cells whose FP64 state that file keeps (16).  The two network-free fillers (collision
 dropped, last exact operator frozen) run the same 200 steps at the same cadence beside it, timed
 on their own.  Inputs are synthetic (qbf.workloads.periodic, seed 3), resident in HBM before the
 timed region.  `pcie_inclusive` adds the host-buffer boundary's per-step copies (state in
