@@ -36,9 +36,11 @@ class QBFError(RuntimeError):
     pass
 
 
-def build(verbose: bool = False) -> str:
-    """Compile csrc/ with hipcc for gfx950 into lib/libqbf.so (works without a GPU)."""
-    out = subprocess.run(["make", "-s", "-C", PKG_DIR], capture_output=not verbose, text=True)
+def build(verbose: bool = False, force: bool = False) -> str:
+    """Compile csrc/ with hipcc for gfx950 into lib/libqbf.so (works without a GPU); force: recompile
+    every source even where the library is up to date."""
+    out = subprocess.run(["make", "-s", *(["-B"] if force else []), "-C", PKG_DIR], capture_output=not verbose,
+                         text=True)
     if out.returncode != 0:
         raise QBFError("build failed:\n" + (out.stdout or "") + (out.stderr or ""))
     return LIB_PATH

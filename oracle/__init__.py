@@ -24,9 +24,9 @@ _lib = None
 _dp = C.POINTER(C.c_double)
 
 
-def build() -> str:
-    """Compile the oracle (gcc, no GPU)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def build(force: bool = False) -> str:
+    """Compile the oracle (gcc, no GPU); force: recompile even where it is up to date."""
+    subprocess.run(["make", "-s", *(["-B"] if force else []), "-C", _HERE], check=True)
     return _LIB_PATH
 
 
