@@ -335,6 +335,83 @@ __global__ __launch_bounds__(T) void k_i3acc(I3Args a)
 	store_native<R, GE>(accp, acc);
 }
 
+// k_padfft with each workgroup looping over pairs of grids (pair b, b + gridDim.x, ...): the state
+// values of its next pair are loaded before the current pair's row passes, so that their latency
+// overlaps the row FFTs and the stores, and one resident workgroup per CU (the LDS exchange takes
+// 141.5 KiB) runs from the first pair to its last.  Same arithmetic in the same order as k_padfft
+// (LDS exchange only).
+template <typename R, int N, int T>
+__global__ __launch_bounds__(T) void k_padfft_loop(PadFFTArgs a)
+{
+	using GE = Geo<2 * N, T>;
+	constexpr int P = GE::P, E = GE::E, G = GE::G;
+	constexpr int NZ = E / 2;                      // non-zero elements per lane in the column pass
+	__shared__ cx<R> lds[Xpose<R, GE>::SIZE];
+	__shared__ cx<R> tw[TwTab<P, T>::SIZE];
+	load_twiddles<R, P, T>(tw, a.tw256);
+	const int line = threadIdx.x / G, p = threadIdx.x % G;
+	const int kn = line % N;
+	const int npair = (a.ngrid + 1) / 2;
+	Xpose<R, GE> X{lds};
+	cx<R> v[E];
+	double2 raw[NZ];
+	// the state values of pair b for this lane (the grid o = 2 b + line / N, if it exists)
+	auto load = [&](int b) {
+		const int o = 2 * b + line / N;
+		if (o < a.ngrid) {
+			const int slot = o % a.nslot;
+			const int cell = o / (a.nslot * a.nq);
+			const double2* src = a.src + ((size_t)cell * a.nsrc + a.srcOfSlot[slot]) * (N * N);
+#pragma unroll
+			for (int j = 0; j < NZ; j++) {
+				const int s = j < NZ / 2 ? j : j + NZ;
+				raw[j] = src[(p + G * s - (s >= E / 4 ? N : 0)) * N + kn];
+			}
+		}
+	};
+	int b = blockIdx.x;
+	load(b);
+	for (; b < npair; b += gridDim.x) {
+		const int o0 = 2 * b;
+		const int o = o0 + line / N;
+		__syncthreads();                           // the previous pair's row passes are done with X
+		if (o < a.ngrid) {
+			const int qq = (o / a.nslot) % a.nq;
+			const size_t poff = (size_t)(a.q0 + qq) * (N * N);
+#pragma unroll
+			for (int s = 0; s < E; s++) {
+				if (padded_nonzero<E>(s)) {
+					const int j = s < E / 4 ? s : s - NZ;
+					const int e = (p + G * s - (s >= E / 4 ? N : 0)) * N + kn;
+					double2 x = raw[j];
+					if (a.preKind == 1) {
+						const double r = ((const double*)a.pre)[poff + e];
+						x.x *= r;
+						x.y *= r;
+					} else if (a.preKind == 2) {
+						const double2 c = ((const double2*)a.pre)[poff + e];
+						x = make_double2(c.x * x.x - c.y * x.y, c.x * x.y + c.y * x.x);
+					}
+					v[s] = mk<R>((R)x.x, (R)x.y);
+				} else {
+					v[s] = czero<R>();
+				}
+			}
+			line_fwd<R, P, G, E>(v, p, tw);
+			X.put_Cb(v, line, p);
+		}
+		__syncthreads();
+		if (b + (int)gridDim.x < npair) load(b + gridDim.x);
+		for (int g = 0; g < 2 && o0 + g < a.ngrid; g++) {
+#pragma unroll
+			for (int s = 0; s < E; s++)
+				v[s] = padded_nonzero<E>(s) ? X.get(line, g * N + p + G * s - (s >= E / 4 ? N : 0)) : czero<R>();
+			line_fwd<R, P, G, E>(v, p, tw);
+			store_native<R, GE>((cx<R>*)a.out + (size_t)(o0 + g) * (P * P), v);
+		}
+	}
+}
+
 // ---------------------------------------------------------------------------------------------
 template <typename R, int N, int T, bool XG>
 __global__ __launch_bounds__(T) void k_combine(CombineArgs a)
