@@ -165,7 +165,8 @@ class CNN:
 
     def update_offset(self, D):
         """Set the offset field to the values of D: written in place into the current field when it
-        has D's shape (graphs captured with that field then see the new values), else set_offset(D)."""
+        has D's shape (graphs captured with that field then see the new values), else set_offset(D)
+        (graphs captured before keep reading the old field)."""
         cur = getattr(self, "_offset", None)
         if cur is None or cur.shape != D.shape or cur.dtype != D.dtype:
             self.set_offset(D)

@@ -214,7 +214,7 @@ class SurrogateStepper:
         if not self.graphs:
             return eager(out)
         if out is None:
-            out = W.new_empty(W.shape)
+            raise QBFError("graph mode steps between fixed buffers: pass the output buffer (out=...)")
         key = (kind, W.data_ptr(), out.data_ptr(), W.shape[0])
         g = self._graphs.get(key)
         if g is not None:

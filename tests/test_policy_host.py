@@ -158,3 +158,10 @@ def test_fillers_never_capture():
     e = FakeEngine()
     st = S.SurrogateStepper(e, None, 1.0, 1.0, 6.0, 0.1, 1e-3, exact_every=3, filler="frozen", graphs=True)
     assert not st.graphs
+
+
+def test_graph_mode_needs_the_output_buffer():
+    e = FakeEngine()
+    st = S.SurrogateStepper(e, FakeNet(e), 1.0, 1.0, 6.0, 0.1, 1e-3, exact_every=2, graphs=True)
+    with pytest.raises(S.QBFError):
+        st.step(torch.zeros(2, 4, 2, 2, dtype=torch.float64))
