@@ -111,10 +111,17 @@ __global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ out, i
 // A workgroup of CROWS waves computes CROWS consecutive row segments (64 pixels each: S / 64
 // per image row); the layer's weights are staged once in LDS (row pitch KC + 4 elements: the 16
 // lanes of a 16-byte read cover all 64 banks but one pair).  Each wave holds a 64-pixel x NC
-// accumulator (4 x NC/16 MFMA tiles): per tap and 32-channel block it loads 4 A fragments from
-// global memory and NC/16 B fragments from LDS for 4 * NC/16 MFMAs.
+// accumulator (4 x NC/16 MFMA tiles): per tap and 32-channel block it loads 4 A fragments
+// and NC/16 B fragments from LDS for 4 * NC/16 MFMAs.
+// ROWS (S = 64, forward, dilation 1): the workgroup's CROWS rows are consecutive rows of one image,
+// so the CROWS + 2 input rows their taps read are staged in LDS once (each input row read once from
+// global memory instead of nine times) and the A fragments come from LDS too.  Row r, pixel px,
+// 16-byte channel chunk c sits at chunk slot (r S + px) CPP + (c ^ ((px / PPB) % CPP)) (CPP chunks
+// per pixel, PPB pixels per 256 bytes): the 16 lanes of a fragment read, 16 consecutive pixels,
+// then fall into 16 different 16-byte slots of a 256-byte bank line.  Otherwise the A fragments
+// are loaded from global memory.
 constexpr int CROWS = 8;
-template <int S, int KC, int NC, int DIR, int MODE>
+template <int S, int KC, int NC, int DIR, int MODE, bool ROWS = false>
 __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_conv_mfma(const bf16* __restrict__ in, const bf16* __restrict__ Wt,
 	const float* __restrict__ bias, float* __restrict__ zout, bf16* __restrict__ aout, float* __restrict__ yout,
 	const float* __restrict__ zprev, float oscale, int dil, const float* __restrict__ amp)
@@ -123,7 +130,11 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 	constexpr int MT = SEGP / 16;     // 16-pixel tiles of a segment
 	constexpr int SEG = S / SEGP;     // segments per image row
 	constexpr int KP = KC + 4;        // LDS pitch of a weight row (elements)
+	constexpr int CPP = KC / 8;       // 16-byte channel chunks per pixel
+	constexpr int PPB = 16 / CPP;     // pixels per 256 bytes
+	static_assert(!ROWS || (S == SEGP && DIR == 1 && S % CROWS == 0), "row staging: S = 64, forward");
 	__shared__ __attribute__((aligned(16))) bf16 wl[9 * NC * KP];
+	__shared__ __attribute__((aligned(16))) bf16 xl[ROWS ? (CROWS + 2) * S * KC : 8];
 	const int tile = blockIdx.x * CROWS + (threadIdx.x >> 6);
 	const int b = tile / (S * SEG);
 	const int y = (tile / SEG) % S;
@@ -137,6 +148,16 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 		*(uint2*)d = *(const uint2*)&w;
 		*(uint2*)(d + 4) = *((const uint2*)&w + 1);
 	}
+	if constexpr (ROWS) {
+		// input rows y0 - 1 .. y0 + CROWS of image b (periodic), y0 = the workgroup's first row
+		const int y0 = (blockIdx.x * CROWS) % S;
+		for (int q = threadIdx.x; q < (CROWS + 2) * S * CPP; q += 64 * CROWS) {
+			const int r = q / (S * CPP), px = (q / CPP) % S, c = q % CPP;
+			const int ys = (y0 - 1 + r + S) & (S - 1);
+			const bf16x8 v = *(const bf16x8*)(in + (((size_t)b * S + ys) * S + px) * KC + c * 8);
+			*(bf16x8*)(xl + ((r * S + px) * CPP + (c ^ ((px / PPB) % CPP))) * 8) = v;
+		}
+	}
 	__syncthreads();
 	f32x4 acc[MT][NT];
 #pragma unroll
@@ -149,11 +170,23 @@ __global__ __launch_bounds__(64 * CROWS) __attribute__((amdgpu_waves_per_eu(4)))
 	constexpr int KB = KC / 32, NIT = 9 * KB;
 	auto load_a = [&](int it, bf16x8* a) {
 		const int tap = it / KB, kb = it % KB;
-		const int ys = (y + DIR * dil * (tap / 3 - 1) + 2 * S) & (S - 1);
-		const int sh = DIR * dil * (tap % 3 - 1) + 2 * S;
-		const bf16* rowp = in + ((size_t)b * S + ys) * S * KC + kb * 32 + kq * 8;
+		if constexpr (ROWS) {
+			// LDS row of the tap: the wave's row (y - y0) + tap / 3 (row 0 holds y0 - 1)
+			const int r = (threadIdx.x >> 6) + tap / 3;
+			const int c = kb * 4 + kq;
 #pragma unroll
-		for (int m = 0; m < MT; m++) a[m] = *(const bf16x8*)(rowp + (size_t)((x0 + m * 16 + i16 + sh) & (S - 1)) * KC);
+			for (int m = 0; m < MT; m++) {
+				const int px = (m * 16 + i16 + tap % 3 - 1 + S) & (S - 1);
+				a[m] = *(const bf16x8*)(xl + ((r * S + px) * CPP + (c ^ ((px / PPB) % CPP))) * 8);
+			}
+		} else {
+			const int ys = (y + DIR * dil * (tap / 3 - 1) + 2 * S) & (S - 1);
+			const int sh = DIR * dil * (tap % 3 - 1) + 2 * S;
+			const bf16* rowp = in + ((size_t)b * S + ys) * S * KC + kb * 32 + kq * 8;
+#pragma unroll
+			for (int m = 0; m < MT; m++)
+				a[m] = *(const bf16x8*)(rowp + (size_t)((x0 + m * 16 + i16 + sh) & (S - 1)) * KC);
+		}
 	};
 	bf16x8 a[MT];
 	load_a(0, a);
@@ -701,13 +734,28 @@ static int run_forward_t(qbf_cnn* c, const float* dIn, int B, float iscale, floa
 	hipLaunchKernelGGL(k_to_nhwc<S>, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, c->stream, dIn, c->x[0], B,
 		iscale, amp);
 	const dim3 grid = conv_grid<S>(B), blk(64 * CROWS);
-	hipLaunchKernelGGL((k_conv_mfma<S, CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
-		train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f, c->dil[0], nullptr);
-	for (int l = 1; l < NL - 1; l++)
-		hipLaunchKernelGGL((k_conv_mfma<S, CH, CH, 1, 0>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
-			train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f, c->dil[l], nullptr);
-	hipLaunchKernelGGL((k_conv_mfma<S, CH, CIO, 1, 1>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
-		c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale, c->dil[NL - 1], amp);
+	// the input rows staged in LDS where the taps allow it (S = 64, dilation 1)
+	constexpr bool R = S == SEGP;
+	if (R && c->dil[0] == 1)
+		hipLaunchKernelGGL((k_conv_mfma<S, CIO, CH, 1, 0, R>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
+			train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f, 1, nullptr);
+	else
+		hipLaunchKernelGGL((k_conv_mfma<S, CIO, CH, 1, 0>), grid, blk, 0, c->stream, c->x[0], c->wb[0], c->bias[0],
+			train ? c->z[0] : nullptr, c->x[1], nullptr, nullptr, 1.0f, c->dil[0], nullptr);
+	for (int l = 1; l < NL - 1; l++) {
+		if (R && c->dil[l] == 1)
+			hipLaunchKernelGGL((k_conv_mfma<S, CH, CH, 1, 0, R>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
+				train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f, 1, nullptr);
+		else
+			hipLaunchKernelGGL((k_conv_mfma<S, CH, CH, 1, 0>), grid, blk, 0, c->stream, c->x[l], c->wb[l], c->bias[l],
+				train ? c->z[l] : nullptr, c->x[l + 1], nullptr, nullptr, 1.0f, c->dil[l], nullptr);
+	}
+	if (R && c->dil[NL - 1] == 1)
+		hipLaunchKernelGGL((k_conv_mfma<S, CH, CIO, 1, 1, R>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
+			c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale, 1, amp);
+	else
+		hipLaunchKernelGGL((k_conv_mfma<S, CH, CIO, 1, 1>), grid, blk, 0, c->stream, c->x[NL - 1], c->wb[NL - 1],
+			c->bias[NL - 1], nullptr, nullptr, dOut, nullptr, oscale, c->dil[NL - 1], amp);
 	HCHK(hipGetLastError());
 	return 0;
 }
