@@ -315,7 +315,10 @@ struct EngineT final : public qbf_engine {
 	DevBuf Wmid, WF, WF2, Wt1, Wt2, src3;
 	DevBuf Wpk, WhatP, reim, Ipk;     // paired path: packed spectra, their padded transforms, realness,
 	                                  // packed I1-type results
-	size_t fBudget = (size_t)2 << 30;
+	// device memory for the F grids of one I3 launch (it sets the quadrature terms per launch; the
+	// accumulators cross HBM once per launch).  Measured at N = 64 on 512 cells: FP32 8 GiB (64 terms
+	// per launch) 329 ms per step against 336 ms with 2 GiB; FP64 16 GiB no faster than 2 GiB.
+	size_t fBudget = std::is_same<R, float>::value ? ((size_t)8 << 30) : ((size_t)2 << 30);
 
 	int init()
 	{
