@@ -42,20 +42,20 @@ METRIC = "collision evals/s at N=64,Nx=512: FP32/surrogate vs FP64 plain, ≤1e-
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12, "bf16": 2.5e15}
 REF_FILE = os.path.join(ROOT, "tests", "golden", "periodic64_fp64_ref.npz")
 
-# HBM traffic of one k_i3acc<float, 64, 1024> launch of this workload (8 quadrature terms x 512
-# cells x 16 K-operator applications): FETCH_SIZE 4.92e6 KiB x 2.000 + WRITE_SIZE 2.62e5 KiB x 1.000,
-# from rocprofv3 --pmc passes over `bench.py --no-cpu-baseline --paths mixed --mixed-steps 1
-# --warmup 0 --copy-calibration`, each counter scaled by its reading for the 256 MiB device copy
-# of the same run.  Algorithmic bytes of the launch: 2.77e9 (F grids 2 GiB, transformed state
-# 256 MiB, accumulator read + write 512 MiB, psi2 0.5 MiB).
-# HBM bytes per k_i3acc launch at N=64, Nx=512 (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated):
-# complex kernel (8 q x 512 cells x 16 applications) and paired kernel (16 q x 512 cells x 8)
-I3_TRAFFIC_BYTES = {False: 10.34e9, True: 5.18e9}
-# N=128 (k_i3acc_big<float>): HBM bytes per K-operator application, from the same two copy-calibrated
-# passes over `bench.py --N 128 --Nx 64 --no-cpu-baseline --paths mixed --mixed-steps 1 --warmup 0
-# --copy-calibration`: FETCH_SIZE 9.84e6 KiB x 2.000 + WRITE_SIZE 1.052e7 KiB x 1.000 per launch of 8192
-# applications (the scratch grids of the line passes go through HBM)
-I3_TRAFFIC_PER_APP_N128 = 3.77e6
+# HBM bytes per K-operator application of the dominant I3 kernel, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, then WRITE_SIZE), each counter scaled by its reading for the 256 MiB device copy of the
+# same run (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the line's `traffic` is this times the average
+# number of applications per launch in the timed region.
+# N=64 (k_i3acc<float, 64, 1024>), `bench.py --no-cpu-baseline --paths mixed,bf16x3,fp64 --mixed-steps 1
+# --fp64-steps 1 --warmup 0 --c3-steps 0 --copy-calibration`: paired launches of 64 quadrature terms x 512
+# cells x 8 applications, FETCH_SIZE 8.97e6 KiB + WRITE_SIZE 2.62e5 KiB = 18.65 GB per launch (algorithmic
+# bytes 8.7 GB: F grids 8 GiB, transformed state 128 MiB, accumulator read + write 512 MiB); the complex
+# kernel was measured with launches of 8 terms x 512 cells x 16 applications (10.34 GB).
+I3_TRAFFIC_PER_APP = {False: 10.34e9 / 65536, True: 18.65e9 / 262144}
+# N=128 (k_i3acc_big<float>), `bench.py --N 128 --Nx 64 --kmax 8 --no-cpu-baseline --paths mixed,fp64
+# --mixed-steps 1 --fp64-steps 1 --warmup 0 --c3-steps 0 --copy-calibration`: 8192 applications per launch
+# (the scratch grids of the line passes go through HBM): complex 30.9 GB, paired 29.9 GB per launch
+I3_TRAFFIC_PER_APP_N128 = {False: 3.77e6, True: 3.65e6}
 
 
 def cpu_baseline(N, J, M, cores, seed):
@@ -492,8 +492,9 @@ def main():
         "roofline": {"bound": "valu", "kernel": i3_kernel,
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp32"], 4),
-                     "traffic": (I3_TRAFFIC_BYTES[bool(used)] if N == 64 else
-                                 round(I3_TRAFFIC_PER_APP_N128 * kapps / max(1, nl)) if N == 128 else None),
+                     "traffic": (round(I3_TRAFFIC_PER_APP[bool(used)] * kapps / max(1, nl)) if N == 64 else
+                                 round(I3_TRAFFIC_PER_APP_N128[bool(used)] * kapps / max(1, nl)) if N == 128
+                                 else None),
                      "launches": nl, "avg_launch_ms": round(kms / max(1, nl), 4),
                      "kernel_share_of_timed": round(kms / 1e3 / twall, 4)},
         "cpu_baseline": cpu,
