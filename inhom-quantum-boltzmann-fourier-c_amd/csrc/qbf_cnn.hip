@@ -458,14 +458,16 @@ __global__ void k_gather_cells(const V* __restrict__ W, const int* __restrict__ 
 	out[i] = W[(size_t)idx[j] * CELL + e];
 }
 
-// res[0] = sum |a (+ D) - b|, res[1] = sum |b|, res[2] = sum |w|  (a and D fp32, b and w fp64;
-// one workgroup of 1024 threads)
-__global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ a, const float* __restrict__ D,
-	const double* __restrict__ b, const double* __restrict__ w, long long n, double* __restrict__ res)
+// res[0] = sum |a (+ D) - b|, res[1] = sum |b|, res[2] = sum |w|  (a and D fp32, b and w fp64), in two
+// launches with a fixed summation order (the same sums bit for bit from run to run): RES_WG workgroups
+// of 256 write partial sums part[wg][3] over a grid-stride range, then one workgroup adds them in order.
+constexpr int RES_WG = 128;
+__global__ __launch_bounds__(256) void k_l1_partial(const float* __restrict__ a, const float* __restrict__ D,
+	const double* __restrict__ b, const double* __restrict__ w, long long n, double* __restrict__ part)
 {
-	__shared__ double sh[3][16];
+	__shared__ double sh[3][4];
 	double d = 0, m = 0, s = 0;
-	for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+	for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
 		d += fabs(op_value(a, D, i) - b[i]);
 		m += fabs(b[i]);
 		s += fabs(w[i]);
@@ -481,16 +483,18 @@ __global__ __launch_bounds__(1024) void k_l1_residual(const float* __restrict__ 
 		sh[2][threadIdx.x >> 6] = s;
 	}
 	__syncthreads();
-	if (threadIdx.x == 0) {
-		double D = 0, M = 0, W = 0;
-		for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-			D += sh[0][k];
-			M += sh[1][k];
-			W += sh[2][k];
-		}
-		res[0] = D;
-		res[1] = M;
-		res[2] = W;
+	if (threadIdx.x < 3) {
+		const int k = threadIdx.x;
+		part[blockIdx.x * 3 + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+	}
+}
+
+__global__ __launch_bounds__(64) void k_l1_final(const double* __restrict__ part, int nwg, double* __restrict__ res)
+{
+	if (threadIdx.x < 3) {
+		double t = 0;
+		for (int g = 0; g < nwg; g++) t += part[g * 3 + threadIdx.x];
+		res[threadIdx.x] = t;
 	}
 }
 
@@ -550,7 +554,7 @@ struct qbf_cnn {
 	const float* off = nullptr;
 	int offCells = 0;                     // the cell count D was given for
 	float* soff = nullptr;                // D on the residual sample
-	double* sres = nullptr;               // three sums of the residual
+	double* sres = nullptr;               // three sums of the residual, then their per-workgroup partials
 	int step = 0;
 	std::vector<void*> allocs;
 	float* alloc_f(size_t n)
@@ -627,7 +631,8 @@ int qbf_cnn_create(int N, int maxBatch, qbf_cnn** out)
 		if (hipMalloc(&p1, qbf_cnn::RMAX * sizeof(int)) == hipSuccess) c->allocs.push_back(p1); else p1 = nullptr;
 		if (hipMalloc(&p2, cplx) == hipSuccess) c->allocs.push_back(p2); else p2 = nullptr;
 		if (hipMalloc(&p3, cplx) == hipSuccess) c->allocs.push_back(p3); else p3 = nullptr;
-		if (hipMalloc(&p4, 3 * sizeof(double)) == hipSuccess) c->allocs.push_back(p4); else p4 = nullptr;
+		// the three sums, then RES_WG partial sums of each
+		if (hipMalloc(&p4, 3 * (1 + RES_WG) * sizeof(double)) == hipSuccess) c->allocs.push_back(p4); else p4 = nullptr;
 		c->sidx = (int*)p1;
 		c->swf = (double*)p2;
 		c->swc = (double*)p3;
@@ -930,8 +935,9 @@ int qbf_surrogate_residual(qbf_cnn* c, qbf_engine* e, const double* dW, int ncel
 		r = run_forward(c, c->sx, nsample, iscale, c->sc, oscale, false);
 	}
 	if (!r) {
-		hipLaunchKernelGGL(k_l1_residual, dim3(1), dim3(1024), 0, c->stream, c->sc, c->off ? c->soff : nullptr, c->sw2,
-			c->sw, n, c->sres);
+		hipLaunchKernelGGL(k_l1_partial, dim3(RES_WG), dim3(256), 0, c->stream, c->sc, c->off ? c->soff : nullptr,
+			c->sw2, c->sw, n, c->sres + 3);
+		hipLaunchKernelGGL(k_l1_final, dim3(1), dim3(64), 0, c->stream, c->sres + 3, RES_WG, c->sres);
 		if (hipMemcpyAsync(h, c->sres, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
 			hipStreamSynchronize(c->stream) != hipSuccess)
 			r = cnn_err(-2, "residual read-back failed");
