@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI of libqbf.so) with the reference's fixtures and the CPU oracle.
 
-Tolerance: the north-star bar for the FP64 path is 1e-12 L1-relative against the oracle; the GPU
+Tolerance: the FP64 path must match the oracle to 1e-12 L1-relative; the GPU
 evaluates an exactly reassociated formulation, so only the rounding differs.  FP32 kernels are
 checked at 2e-5 L1-relative.
 """
