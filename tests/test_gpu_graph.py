@@ -13,7 +13,8 @@ torch = pytest.importorskip("torch")
 import qbf  # noqa: E402
 from qbf import workloads as wl  # noqa: E402
 
-CASES = [(32, qbf.FP64, 0), (64, qbf.FP32, 2), (64, qbf.FP32, 0), (128, qbf.FP32, 0), (128, qbf.FP32, 2)]
+CASES = [(32, qbf.FP64, 0), (64, qbf.FP32, 2), (64, qbf.FP32, 0), (64, qbf.BF16X3, 2), (128, qbf.FP32, 0),
+         (128, qbf.FP32, 2)]
 
 
 @pytest.mark.parametrize("N,prec,pairs", CASES)

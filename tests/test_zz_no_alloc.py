@@ -21,20 +21,21 @@ def _frames(e, W0, L, h, dt, nsteps, noalloc):
     return rc, out
 
 
-def test_noalloc_loop_equals_allocating_loop():
+@pytest.mark.parametrize("N,prec", [(32, "FP64"), (64, "FP32")])
+def test_noalloc_loop_equals_allocating_loop(N, prec):
     import torch
     import oracle as O
     import qbf
     from qbf import workloads as wl
-    N, J, M, L = 32, 4, 2, 6.0
+    J, M, L = 4, 2, 6.0
     q = O.Quad(N, J, L, 3.75, M)
-    e = qbf.Engine(N, qbf.FP64, max_cells=16)
+    e = qbf.Engine(N, getattr(qbf, prec), max_cells=16)
     e.set_quadrature(q.I1, q.I2, q.I3, q.I4)
     e.set_moment_correction(qbf.CONS_ALL)
     for ncell, nsteps in ((16, 4), (8, 3)):
         W0 = torch.from_numpy(wl.periodic(N, ncell, seed=ncell)).cuda()
         h, dt = 1.0 / ncell, wl.cfl_dt(N, ncell)
-        print(f"ncell={ncell} steps={nsteps}: allocating loop, then the allocation-free one", flush=True)
+        print(f"N={N} {prec} ncell={ncell} steps={nsteps}: allocating loop, then the allocation-free one", flush=True)
         if ncell == 16:
             rc_a, ref = _frames(e, W0, L, h, dt, nsteps, noalloc=False)
             rc_b, got = _frames(e, W0, L, h, dt, nsteps, noalloc=True)
